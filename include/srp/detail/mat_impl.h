@@ -26,7 +26,7 @@ SRP_MATH_FN mat4 mat4MultiplyMat4(const mat4* SRP_RESTRICT a, const mat4* SRP_RE
 	return r;
 }
 
-SRP_MATH_FN mat4 srpMat4Diag_(float a, float b, float c, float d)
+SRP_HD static inline mat4 srpMat4Diag_(float a, float b, float c, float d)
 {
 	mat4 r;
 	for (int i = 0; i < 4; i++)
@@ -48,9 +48,9 @@ SRP_MATH_FN mat4 mat4ConstructTranslate(float x, float y, float z)
 
 SRP_MATH_FN mat4 mat4ConstructRotate(float x, float y, float z)
 {
-	const double sx = sin(x), cx = cos(x);
-	const double sy = sin(y), cy = cos(y);
-	const double sz = sin(z), cz = cos(z);
+	const double sx = sin((double) x), cx = cos((double) x);
+	const double sy = sin((double) y), cy = cos((double) y);
+	const double sz = sin((double) z), cz = cos((double) z);
 	mat4 r = srpMat4Diag_(0, 0, 0, 1);
 	r.data[0][0] = cy * cz;
 	r.data[0][1] = sx * sy * cz - cx * sz;

@@ -1,0 +1,182 @@
+/* Device-side data layouts shared by libsrp_amd and HIP shader modules.
+ *
+ * The library runs the shader-independent stages (assembly, clipping,
+ * setup, binning, sorting) and writes the records below; the kernels that
+ * call user shaders (vertex stage, rasteriser, fragment lists) are
+ * instantiated inside the shader module from srp/device/pipeline.hpp and
+ * read the same records.  SRP_ABI_VERSION guards against mixing builds. */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+#include "srp/context.h"
+
+#define SRP_ABI_VERSION 3u
+#define SRP_MAX_VARYINGS 16
+#define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
+#define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
+
+/* interpolation kinds of a 4-byte word in the fast path */
+enum SrpWordKind
+{
+	SRP_WORD_PERSPECTIVE = 0,  /* float, perspective-correct */
+	SRP_WORD_AFFINE = 1,       /* float, screen-space linear */
+	SRP_WORD_PROVOKING = 2,    /* flat float or integer bytes: provoking vertex */
+	SRP_WORD_UNUSED = 3        /* padding beyond the described varyings */
+};
+
+typedef struct SrpVaryingDesc
+{
+	uint32_t nItems;
+	uint32_t type;    /* SRPType */
+	uint32_t mode;    /* SRPInterpolationMode */
+	uint32_t offset;  /* byte offset of the first element */
+} SrpVaryingDesc;
+
+/* Render state captured when the draw is issued. */
+typedef struct SrpState
+{
+	uint32_t provokingLast;
+	uint32_t cullFace, frontFace, polygonMode;
+	float pointSize;
+	uint32_t scissorEnabled;
+	uint64_t sx0, sx1, sy0, sy1;      /* scissor box, half-open */
+	uint32_t stencilEnabled;
+	SRPStencilFaceState stencil[2];   /* [0] front, [1] back */
+	uint32_t depthTest, depthWrite, depthOp;
+	uint32_t lateZ;                   /* fragment shader may overwrite depth */
+} SrpState;
+
+/* Triangle after clipping and setup (one per assembled primitive). */
+typedef struct SrpTri
+{
+	float l0[3];           /* barycentrics at the bbox's first pixel centre */
+	float dx[3], dy[3];    /* per-pixel and per-row increments */
+	uint16_t x0, y0, x1, y1; /* bounding box [x0,x1) x [y0,y1) */
+	uint32_t flags;        /* bits 0-2: top-left edge; bit 3: front facing */
+	uint32_t id;           /* primitive ID */
+	float invW[3];
+	float zw[3];           /* ndc z * invW per vertex */
+	uint32_t vref[3];      /* varying slots after the winding fix */
+	uint32_t pad;
+} SrpTri;
+
+typedef struct SrpLine
+{
+	float ssx[2], ssy[2];
+	float invW[2];
+	float zw[2];
+	uint32_t vref[2];
+	uint32_t id;
+	uint32_t pad;
+} SrpLine;
+
+typedef struct SrpPoint
+{
+	float ssx, ssy;
+	float z, w;            /* ndc z and w (w is 1 after the divide) */
+	uint32_t vref;
+	uint32_t id;
+	uint32_t pad[2];
+} SrpPoint;
+
+/* One line or point fragment, in primitive order. */
+typedef struct SrpFrag
+{
+	int32_t px, py;        /* unwrapped pixel (a line may round to x == W) */
+	uint32_t prim;         /* record index */
+	float t;               /* line parameter (lines only) */
+} SrpFrag;
+
+typedef struct SrpTexDesc
+{
+	const uint32_t* texels;  /* RGBA8, row-major, R in the low byte */
+	int32_t w, h, wM1, hM1;
+	int32_t wrapX, wrapY;
+} SrpTexDesc;
+
+/* Everything a draw's kernels read, in one device block. */
+typedef struct SrpDrawParams
+{
+	SrpState st;
+	uint32_t width, height;
+	uint32_t* color;
+	float* depth;
+	uint8_t* stencil;
+	const uint8_t* uniform;
+
+	/* vertex stage */
+	const uint8_t* vertices;
+	uint64_t vertexStride;
+	uint64_t cacheBase;      /* vertex index of cache slot 0 */
+	uint32_t cacheSize;
+	float* clip;             /* 4 floats per cache slot */
+	uint8_t* varyings;       /* varyings of the cache slots */
+	uint8_t* genVaryings;    /* varyings of clip-generated vertices; slot
+	                            cacheSize + k is genVaryings[k] */
+	uint32_t varyingsSize;
+	uint32_t describedSize;
+	uint32_t nVaryings;
+	SrpVaryingDesc var[SRP_MAX_VARYINGS];
+	uint32_t wordPath;       /* 1: every element 4-byte and aligned */
+	uint32_t nWords;
+	uint8_t wordKind[SRP_MAX_WORDS];
+
+	/* tiles */
+	uint32_t tileShift, tilesX, tilesY;
+
+	unsigned long long* stats;   /* covered, shaded, primitives, draws */
+} SrpDrawParams;
+
+/* Launch arguments of the shader-module kernels. */
+typedef struct SrpVsLaunch
+{
+	const SrpDrawParams* params;
+	uint32_t count;
+} SrpVsLaunch;
+
+typedef struct SrpRasterLaunch
+{
+	const SrpDrawParams* params;
+	const SrpTri* tris;
+	const uint32_t* tileList;
+	const uint32_t* tileStart;  /* nTiles + 1 */
+	uint32_t nTiles;
+	uint32_t lateZ;
+	uint32_t words;             /* fast interpolation path usable */
+	uint32_t tile;              /* tile side: 8 or 16 */
+	uint32_t nBlocks;           /* grid size (for the XCD-aware tile order) */
+} SrpRasterLaunch;
+
+typedef struct SrpFragLaunch
+{
+	const SrpDrawParams* params;
+	const SrpFrag* frags;        /* in primitive order */
+	const uint32_t* keys;        /* pixel index per fragment, sorted (stable) */
+	const uint32_t* order;       /* fragment index for each sorted key */
+	const SrpLine* lines;        /* kind 0 */
+	const SrpPoint* points;      /* kind 1 */
+	uint32_t count;
+	uint32_t kind;
+	uint32_t lateZ;
+	uint32_t words;
+} SrpFragLaunch;
+
+typedef void (*SrpVsLaunchFn)(const SrpVsLaunch*, void* stream);
+typedef void (*SrpRasterLaunchFn)(const SrpRasterLaunch*, void* stream);
+typedef void (*SrpFragLaunchFn)(const SrpFragLaunch*, void* stream);
+
+typedef struct SrpVsEntry
+{
+	uint32_t abiVersion;
+	size_t uniformSize;
+	SrpVsLaunchFn launch;
+} SrpVsEntry;
+
+typedef struct SrpFsEntry
+{
+	uint32_t abiVersion;
+	size_t uniformSize;
+	SrpRasterLaunchFn raster;
+	SrpFragLaunchFn frags;
+} SrpFsEntry;

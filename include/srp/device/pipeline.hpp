@@ -1,0 +1,560 @@
+/* Kernels that call user shaders, instantiated inside each HIP shader
+ * module by the SRP_HIP_*_SHADER macros (srp/srp_device.h):
+ *
+ *   srp_vs_kernel      vertex shader over the draw's post-VS cache slots
+ *   srp_raster_kernel  sort-middle tile rasteriser for triangles
+ *   srp_frag_kernel    per-pixel ordered fragment lists (lines, points)
+ *
+ * Numerics follow the reference renderer bit for bit (kitrofimov/srp):
+ * FP contraction is off for this whole header, float division and sqrt
+ * are correctly rounded, the barycentric walk replays the reference's
+ * sequence of float additions, and every per-fragment test runs in the
+ * reference's order per pixel and per primitive. */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "srp/shaders.h"
+#include "srp/device/abi.h"
+
+#pragma clang fp contract(off)
+
+namespace srp {
+namespace dev {
+
+/* reference math/utils.h:41-45 (comparison in double precision) */
+__device__ __forceinline__ bool roughlyZero(float x) { return fabs((double) x) <= 1e-9; }
+
+/* reference raster/fragment.c:156-178, :225-245 */
+template <typename T>
+__device__ __forceinline__ bool compare(uint32_t op, T lhs, T rhs)
+{
+	switch (op)
+	{
+		case SRP_COMPARE_NEVER: return false;
+		case SRP_COMPARE_ALWAYS: return true;
+		case SRP_COMPARE_LESS: return lhs < rhs;
+		case SRP_COMPARE_LEQUAL: return lhs <= rhs;
+		case SRP_COMPARE_GREATER: return lhs > rhs;
+		case SRP_COMPARE_GEQUAL: return lhs >= rhs;
+		case SRP_COMPARE_EQUAL: return lhs == rhs;
+		case SRP_COMPARE_NOTEQUAL: return lhs != rhs;
+		default: return false;
+	}
+}
+
+/* reference raster/fragment.c:180-223: op result merged under the write mask */
+__device__ __forceinline__ uint32_t stencilApply(uint32_t op, uint32_t s, const SRPStencilFaceState& st)
+{
+	uint32_t v;
+	switch (op)
+	{
+		case SRP_STENCIL_KEEP: v = s; break;
+		case SRP_STENCIL_ZERO: v = 0; break;
+		case SRP_STENCIL_REPLACE: v = st.ref; break;
+		case SRP_STENCIL_INCR: v = (s < 255u) ? s + 1u : 255u; break;
+		case SRP_STENCIL_INCR_WRAP: v = (s + 1u) & 255u; break;
+		case SRP_STENCIL_DECR: v = (s > 0u) ? s - 1u : 0u; break;
+		case SRP_STENCIL_DECR_WRAP: v = (s - 1u) & 255u; break;
+		case SRP_STENCIL_INVERT: v = (~s) & 255u; break;
+		default: v = s; break;
+	}
+	const uint32_t m = st.writeMask;
+	return (s & ~m & 255u) | (v & m);
+}
+
+/* reference core/color.c:14-23: clamp(c*255) to [0,255], truncate; R in the top byte. */
+__device__ __forceinline__ uint32_t packColor(const float c[4])
+{
+	uint32_t r = 0;
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+	{
+		const float v = c[i] * 255.0f;
+		const float cl = (v < 0.0f) ? 0.0f : ((v > 255.0f) ? 255.0f : v);
+		const uint32_t b = (cl != cl) ? 0u : (uint32_t) cl;
+		r |= b << (24 - 8 * i);
+	}
+	return r;
+}
+
+/* The reference walks the bounding box adding dldx per pixel and dldy per
+ * row (raster/triangle.c:101-109); `n` additions of `d` in that order. */
+__device__ __forceinline__ float replay(float a, float d, int n)
+{
+	for (int k = 0; k < n; k++)
+		a = a + d;
+	return a;
+}
+
+/* ---------------------------------------------------------------- varyings */
+
+/* Varying bytes of a vertex slot: post-VS cache or clip-generated vertex. */
+__device__ __forceinline__ const uint8_t* varyingPtr(const SrpDrawParams* P, uint32_t vref)
+{
+	return (vref < P->cacheSize) ? P->varyings + (size_t) vref * P->varyingsSize
+	                             : P->genVaryings + (size_t) (vref - P->cacheSize) * P->varyingsSize;
+}
+
+__device__ __forceinline__ float loadF(const uint8_t* p)
+{
+	float v;
+	__builtin_memcpy(&v, p, 4);
+	return v;
+}
+
+__device__ __forceinline__ double loadD(const uint8_t* p)
+{
+	double v;
+	__builtin_memcpy(&v, p, 8);
+	return v;
+}
+
+__device__ __forceinline__ uint32_t elemSize(uint32_t type)
+{
+	switch (type)
+	{
+		case SRP_FLOAT: case SRP_INT32: case SRP_UINT32: return 4;
+		case SRP_DOUBLE: case SRP_INT64: case SRP_UINT64: return 8;
+		case SRP_INT8: case SRP_UINT8: return 1;
+		case SRP_INT16: case SRP_UINT16: return 2;
+		default: return 0;
+	}
+}
+
+/* Fast path: every element is a 4-byte word at a 4-byte aligned offset.
+ * reference pipeline/interpolation.c:63-91 per word: perspective
+ * ((A*invW)*w summed, times 1/sum(invW*w)), affine (A*w summed) or a copy
+ * from the provoking vertex.  `out` stays in registers when the shader's
+ * reads have constant offsets. */
+template <int N>
+__device__ __forceinline__ void interpWords(const SrpDrawParams* P, const uint32_t* const* v, const float* w,
+                                            const float* invW, float rec, uint32_t pv, uint32_t* out)
+{
+#pragma unroll
+	for (int k = 0; k < SRP_MAX_WORDS; k++)
+	{
+		if (k >= (int) P->nWords)
+		{
+			out[k] = 0u;
+			continue;
+		}
+		const uint32_t kind = P->wordKind[k];
+		if (kind == SRP_WORD_PERSPECTIVE)
+		{
+			float val = 0.0f;
+#pragma unroll
+			for (int i = 0; i < N; i++)
+				val += __uint_as_float(v[i][k]) * invW[i] * w[i];
+			val *= rec;
+			out[k] = __float_as_uint(val);
+		}
+		else if (kind == SRP_WORD_AFFINE)
+		{
+			float val = 0.0f;
+#pragma unroll
+			for (int i = 0; i < N; i++)
+				val += __uint_as_float(v[i][k]) * w[i];
+			out[k] = __float_as_uint(val);
+		}
+		else if (kind == SRP_WORD_PROVOKING)
+			out[k] = v[pv][k];
+		else
+			out[k] = 0u;
+	}
+}
+
+/* General path: any layout, byte addressed (interpolation.c:93-163). */
+template <int N>
+__device__ inline void interpBytes(const SrpDrawParams* P, const uint8_t* const* v, const float* w,
+                                   const float* invW, float rec, uint32_t pv, uint8_t* out)
+{
+	for (uint32_t a = 0; a < P->nVaryings; a++)
+	{
+		const SrpVaryingDesc d = P->var[a];
+		const bool persp = d.mode == SRP_INTERPOLATION_MODE_PERSPECTIVE;
+		const bool affine = d.mode == SRP_INTERPOLATION_MODE_AFFINE;
+		if (d.type == SRP_FLOAT)
+		{
+			for (uint32_t e = 0; e < d.nItems; e++)
+			{
+				const uint32_t off = d.offset + 4 * e;
+				float val = 0.0f;
+				if (persp)
+				{
+					for (int i = 0; i < N; i++) val += loadF(v[i] + off) * invW[i] * w[i];
+					val *= rec;
+				}
+				else if (affine)
+					for (int i = 0; i < N; i++) val += loadF(v[i] + off) * w[i];
+				else
+					val = loadF(v[pv] + off);
+				__builtin_memcpy(out + off, &val, 4);
+			}
+		}
+		else if (d.type == SRP_DOUBLE)
+		{
+			for (uint32_t e = 0; e < d.nItems; e++)
+			{
+				const uint32_t off = d.offset + 8 * e;
+				double val = 0.0;
+				if (persp)
+				{
+					for (int i = 0; i < N; i++) val += loadD(v[i] + off) * (double) invW[i] * (double) w[i];
+					val *= (double) rec;
+				}
+				else if (affine)
+					for (int i = 0; i < N; i++) val += loadD(v[i] + off) * (double) w[i];
+				else
+					val = loadD(v[pv] + off);
+				__builtin_memcpy(out + off, &val, 8);
+			}
+		}
+		else
+		{
+			const uint32_t n = elemSize(d.type) * d.nItems;
+			for (uint32_t b = 0; b < n; b++) out[d.offset + b] = v[pv][d.offset + b];
+		}
+	}
+}
+
+/* --------------------------------------------------------- fragment ops */
+
+struct PixelState
+{
+	uint32_t color;
+	float depth;
+	uint32_t stencil;
+	uint32_t covered;
+	uint32_t shaded;
+};
+
+/* reference raster/fragment.c:63-125, for one fragment of pixel (px, py)
+ * whose fb values live in `ps`.  `interp(buf)` fills the varying buffer and
+ * returns the pointer handed to the shader; it runs only when the fragment
+ * reaches the shader (interpolation has no side effects). */
+template <class FS, bool LATE, class Interp>
+__device__ __forceinline__ void fragment(const SrpDrawParams* P, int px, int py, bool front, float depth,
+                                         float fc3, uint32_t primID, PixelState& ps, Interp&& interp)
+{
+	const SrpState& st = P->st;
+	ps.covered++;
+	if (st.scissorEnabled)
+	{
+		const uint64_t ux = (uint64_t) (int64_t) px, uy = (uint64_t) (int64_t) py;
+		if (ux < st.sx0 || ux >= st.sx1 || uy < st.sy0 || uy >= st.sy1)
+			return;
+	}
+	const SRPStencilFaceState& sf = st.stencil[front ? 0 : 1];
+	const float stored = ps.depth;
+	if (st.stencilEnabled && !compare<uint32_t>(sf.func, (uint32_t) (sf.ref & sf.mask), ps.stencil & sf.mask))
+	{
+		ps.stencil = stencilApply(sf.sfailOp, ps.stencil, sf);
+		return;
+	}
+	if (!LATE && st.depthTest && !compare<float>(st.depthOp, depth, stored))
+	{
+		if (st.stencilEnabled) ps.stencil = stencilApply(sf.dfailOp, ps.stencil, sf);
+		return;
+	}
+	SRPFragmentShaderIn in;
+	in.uniform = (SRPUniform*) P->uniform;
+	in.fragCoord[0] = (float) ((double) px + 0.5);
+	in.fragCoord[1] = (float) ((double) py + 0.5);
+	in.fragCoord[2] = depth;
+	in.fragCoord[3] = fc3;
+	in.frontFacing = front;
+	in.primitiveID = primID;
+	in.varyings = interp();
+	SRPFragmentShaderOut out;
+	out.color[0] = 0.0f; out.color[1] = 0.0f; out.color[2] = 0.0f; out.color[3] = 0.0f;
+	out.fragDepth = __builtin_nanf("");
+	ps.shaded++;
+	FS()(&in, &out);
+	if (LATE)
+	{
+		if (!isnan(out.fragDepth)) depth = out.fragDepth;
+		if (st.depthTest && !compare<float>(st.depthOp, depth, stored))
+		{
+			if (st.stencilEnabled) ps.stencil = stencilApply(sf.dfailOp, ps.stencil, sf);
+			return;
+		}
+	}
+	if (st.stencilEnabled) ps.stencil = stencilApply(sf.passOp, ps.stencil, sf);
+	ps.color = packColor(out.color);
+	if (st.depthTest && st.depthWrite)
+		ps.depth = depth;
+}
+
+__device__ __forceinline__ void flushStats(const SrpDrawParams* P, uint32_t covered, uint32_t shaded)
+{
+	/* wave-level reduction, one atomic per wave and counter */
+	for (int off = 32; off > 0; off >>= 1)
+	{
+		covered += __shfl_xor(covered, off);
+		shaded += __shfl_xor(shaded, off);
+	}
+	if ((threadIdx.x & 63) == 0 && (covered | shaded))
+	{
+		atomicAdd(&P->stats[0], (unsigned long long) covered);
+		atomicAdd(&P->stats[1], (unsigned long long) shaded);
+	}
+}
+
+/* ------------------------------------------------------------- kernels */
+
+/* Vertex stage: one thread per post-VS cache slot (vertex index
+ * cacheBase + i).  reference pipeline/vertex_processing.c:55-74. */
+template <class VS>
+__global__ __launch_bounds__(256) void srp_vs_kernel(SrpVsLaunch L)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= L.count)
+		return;
+	const SrpDrawParams* P = L.params;
+	const uint64_t vi = P->cacheBase + i;
+	SRPVertexShaderIn in;
+	in.uniform = (SRPUniform*) P->uniform;
+	in.vertex = (SRPVertex*) (P->vertices + vi * P->vertexStride);
+	in.vertexID = (size_t) vi;
+	SRPVertexShaderOut out;
+	out.clipPosition[0] = 0.0f; out.clipPosition[1] = 0.0f;
+	out.clipPosition[2] = 0.0f; out.clipPosition[3] = 0.0f;
+	out.varyings = (SRPVarying*) (P->varyings + (size_t) i * P->varyingsSize);
+	VS()(&in, &out);
+	float4 c;
+	c.x = out.clipPosition[0]; c.y = out.clipPosition[1];
+	c.z = out.clipPosition[2]; c.w = out.clipPosition[3];
+	reinterpret_cast<float4*>(P->clip)[i] = c;
+}
+
+/* Blocks are issued round-robin over the 8 XCDs; give each XCD a
+ * contiguous run of tiles so that primitives shared by neighbouring tiles
+ * hit the same L2. */
+__device__ __forceinline__ uint32_t xcdOrder(uint32_t b, uint32_t nBlocks)
+{
+	const uint32_t q = nBlocks / 8u, r = nBlocks % 8u;
+	const uint32_t x = b % 8u, k = b / 8u;
+	return x * q + (x < r ? x : r) + k;
+}
+
+/* Triangle rasteriser: each lane owns one pixel of a TILE x TILE tile and
+ * keeps its colour/depth/stencil in registers while it walks the tile's
+ * primitive list in primitive order (so per-pixel API order holds).  The
+ * barycentrics of a lane are replayed from the bbox origin exactly as the
+ * reference's scan would have produced them. */
+template <class FS, bool LATE, bool WORDS, int TILE>
+__global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
+{
+	constexpr int TILES_PER_BLOCK = (TILE == 8) ? 4 : 1;
+	const SrpDrawParams* P = L.params;
+	const uint32_t t = threadIdx.x;
+	const uint32_t local = (TILE == 8) ? (t & 63u) : t;
+	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks) * TILES_PER_BLOCK + ((TILE == 8) ? (t >> 6) : 0u);
+	if (tile >= L.nTiles)
+		return;
+	const uint32_t begin = L.tileStart[tile], end = L.tileStart[tile + 1];
+	if (begin == end)
+		return;
+	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;
+	const int px = (int) (tx * TILE + local % TILE);
+	const int py = (int) (ty * TILE + local / TILE);
+	const bool inFb = px < (int) P->width && py < (int) P->height;
+	const size_t idx = (size_t) py * P->width + (size_t) px;
+
+	PixelState ps;
+	ps.color = 0; ps.depth = 0.0f; ps.stencil = 0; ps.covered = 0; ps.shaded = 0;
+	if (inFb)
+	{
+		ps.color = P->color[idx];
+		ps.depth = P->depth[idx];
+		ps.stencil = P->stencil[idx];
+	}
+	const uint32_t pv = P->st.provokingLast ? 2u : 0u;
+
+	for (uint32_t i = begin; i < end; i++)
+	{
+		const SrpTri* T = &L.tris[L.tileList[i]];
+		if (!(px >= (int) T->x0 && px < (int) T->x1 && py >= (int) T->y0 && py < (int) T->y1))
+			continue;
+		const int ny = py - (int) T->y0, nx = px - (int) T->x0;
+		float l[3];
+		bool inside = true;
+#pragma unroll
+		for (int e = 0; e < 3; e++)
+		{
+			l[e] = replay(replay(T->l0[e], T->dy[e], ny), T->dx[e], nx);
+			const bool tl = (T->flags >> e) & 1u;
+			inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
+		}
+		if (!inside)
+			continue;
+		const float iw[3] = {T->invW[0], T->invW[1], T->invW[2]};
+		const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
+		const float depth = T->zw[0] * l[0] + T->zw[1] * l[1] + T->zw[2] * l[2];
+		const bool front = (T->flags >> 3) & 1u;
+		if (WORDS)
+		{
+			uint32_t buf[SRP_MAX_WORDS];
+			fragment<FS, LATE>(P, px, py, front, depth, rec, T->id, ps, [&]() {
+				const uint32_t* v[3];
+#pragma unroll
+				for (int k = 0; k < 3; k++)
+					v[k] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T->vref[k]));
+				interpWords<3>(P, v, l, iw, rec, pv, buf);
+				return (SRPInterpolated*) buf;
+			});
+		}
+		else
+		{
+			uint8_t buf[SRP_MAX_VARYING_BYTES];
+			fragment<FS, LATE>(P, px, py, front, depth, rec, T->id, ps, [&]() {
+				const uint8_t* v[3];
+#pragma unroll
+				for (int k = 0; k < 3; k++) v[k] = varyingPtr(P, T->vref[k]);
+				for (uint32_t b = 0; b < P->describedSize; b++) buf[b] = 0;
+				interpBytes<3>(P, v, l, iw, rec, pv, buf);
+				return (SRPInterpolated*) buf;
+			});
+		}
+	}
+	if (inFb)
+	{
+		P->color[idx] = ps.color;
+		P->depth[idx] = ps.depth;
+		P->stencil[idx] = (uint8_t) ps.stencil;
+	}
+	flushStats(P, ps.covered, ps.shaded);
+}
+
+/* Lines and points: fragments sorted (stably) by pixel; the first thread
+ * of each pixel's run applies them in primitive order. */
+template <class FS, bool LATE, bool WORDS>
+__global__ __launch_bounds__(256) void srp_frag_kernel(SrpFragLaunch L)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	const SrpDrawParams* P = L.params;
+	PixelState ps;
+	ps.color = 0; ps.depth = 0.0f; ps.stencil = 0; ps.covered = 0; ps.shaded = 0;
+	const uint32_t npix = P->width * P->height;
+	uint32_t key = 0xFFFFFFFFu;
+	bool head = false;
+	if (i < L.count)
+	{
+		key = L.keys[i];
+		head = key < npix && (i == 0 || L.keys[i - 1] != key);
+	}
+	if (head)
+	{
+		ps.color = P->color[key];
+		ps.depth = P->depth[key];
+		ps.stencil = P->stencil[key];
+		const uint32_t pv = P->st.provokingLast ? 1u : 0u;
+		for (uint32_t j = i; j < L.count && L.keys[j] == key; j++)
+		{
+			const SrpFrag f = L.frags[L.order[j]];
+			if (L.kind == 0)
+			{
+				const SrpLine* Ln = &L.lines[f.prim];
+				const float w[2] = {1.0f - f.t, f.t};
+				const float iw[2] = {Ln->invW[0], Ln->invW[1]};
+				const float rec = 1.0f / (iw[0] * w[0] + iw[1] * w[1]);
+				const float depth = Ln->zw[0] * w[0] + Ln->zw[1] * w[1];
+				if (WORDS)
+				{
+					uint32_t buf[SRP_MAX_WORDS];
+					fragment<FS, LATE>(P, f.px, f.py, true, depth, rec, Ln->id, ps, [&]() {
+						const uint32_t* v[2];
+						for (int k = 0; k < 2; k++)
+							v[k] = reinterpret_cast<const uint32_t*>(varyingPtr(P, Ln->vref[k]));
+						interpWords<2>(P, v, w, iw, rec, pv, buf);
+						return (SRPInterpolated*) buf;
+					});
+				}
+				else
+				{
+					uint8_t buf[SRP_MAX_VARYING_BYTES];
+					fragment<FS, LATE>(P, f.px, f.py, true, depth, rec, Ln->id, ps, [&]() {
+						const uint8_t* v[2];
+						for (int k = 0; k < 2; k++) v[k] = varyingPtr(P, Ln->vref[k]);
+						for (uint32_t b = 0; b < P->describedSize; b++) buf[b] = 0;
+						interpBytes<2>(P, v, w, iw, rec, pv, buf);
+						return (SRPInterpolated*) buf;
+					});
+				}
+			}
+			else
+			{
+				/* reference raster/point.c:64-71: varyings passed through */
+				const SrpPoint* Pt = &L.points[f.prim];
+				fragment<FS, LATE>(P, f.px, f.py, true, Pt->z, Pt->w, Pt->id, ps, [&]() {
+					return (SRPInterpolated*) varyingPtr(P, Pt->vref);
+				});
+			}
+		}
+		P->color[key] = ps.color;
+		P->depth[key] = ps.depth;
+		P->stencil[key] = (uint8_t) ps.stencil;
+	}
+	flushStats(P, ps.covered, ps.shaded);
+}
+
+/* ------------------------------------------------------------ launchers */
+
+template <class VS>
+void launchVS(const SrpVsLaunch* L, void* stream)
+{
+	const uint32_t blocks = (L->count + 255u) / 256u;
+	if (blocks)
+		hipLaunchKernelGGL(srp_vs_kernel<VS>, dim3(blocks), dim3(256), 0, (hipStream_t) stream, *L);
+}
+
+template <class FS, bool LATE, bool WORDS>
+void launchRasterT(const SrpRasterLaunch* L, hipStream_t s)
+{
+	if (L->tile == 16)
+		hipLaunchKernelGGL((srp_raster_kernel<FS, LATE, WORDS, 16>), dim3(L->nBlocks), dim3(256), 0, s, *L);
+	else
+		hipLaunchKernelGGL((srp_raster_kernel<FS, LATE, WORDS, 8>), dim3(L->nBlocks), dim3(256), 0, s, *L);
+}
+
+template <class FS>
+void launchRaster(const SrpRasterLaunch* L, void* stream)
+{
+	hipStream_t s = (hipStream_t) stream;
+	if (L->nBlocks == 0) return;
+	if (L->lateZ)
+	{
+		if (L->words) launchRasterT<FS, true, true>(L, s);
+		else launchRasterT<FS, true, false>(L, s);
+	}
+	else
+	{
+		if (L->words) launchRasterT<FS, false, true>(L, s);
+		else launchRasterT<FS, false, false>(L, s);
+	}
+}
+
+template <class FS>
+void launchFrags(const SrpFragLaunch* L, void* stream)
+{
+	hipStream_t s = (hipStream_t) stream;
+	const uint32_t blocks = (L->count + 255u) / 256u;
+	if (!blocks) return;
+	if (L->lateZ)
+	{
+		if (L->words) hipLaunchKernelGGL((srp_frag_kernel<FS, true, true>), dim3(blocks), dim3(256), 0, s, *L);
+		else hipLaunchKernelGGL((srp_frag_kernel<FS, true, false>), dim3(blocks), dim3(256), 0, s, *L);
+	}
+	else
+	{
+		if (L->words) hipLaunchKernelGGL((srp_frag_kernel<FS, false, true>), dim3(blocks), dim3(256), 0, s, *L);
+		else hipLaunchKernelGGL((srp_frag_kernel<FS, false, false>), dim3(blocks), dim3(256), 0, s, *L);
+	}
+}
+
+}  // namespace dev
+}  // namespace srp

@@ -1,0 +1,129 @@
+"""Build recipe for libsrp_amd and HIP shader modules (gfx950).
+
+Every translation unit, host or device, is compiled with FP contraction off
+and without fast-math: bit-exact parity with the reference renderer depends
+on it.  HIP code also keeps correctly rounded float division/sqrt and
+denormals, and disables packed FP32 instructions.
+
+    python -m srp_amd.build            # library + test scene module + oracle
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "srp_amd")
+INCLUDE = os.path.join(REPO, "include")
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build", "obj")
+LIB = os.path.join(PKG, "libsrp_amd.so")
+SCENE_DIR = os.path.join(REPO, "tests", "scenes")
+SCENE_BUILD = os.path.join(SCENE_DIR, "build")
+SCENE_LIB = os.path.join(SCENE_BUILD, "libscenes_amd.so")
+
+ARCH = os.environ.get("SRP_OFFLOAD_ARCH", "gfx950")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CC = os.environ.get("CC", "gcc")
+
+NUMERIC_FLAGS = ["-ffp-contract=off", "-fno-fast-math"]
+HIP_FLAGS = [
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", "-Wno-pass-failed",
+] + NUMERIC_FLAGS
+C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-Wall", "-Wno-unused-function"] + NUMERIC_FLAGS
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}")
+    return r.stdout
+
+
+def _hip_obj(src, obj, includes):
+    inc = sum((["-I", d] for d in includes), [])
+    _run([HIPCC] + HIP_FLAGS + inc + ["-c", src, "-o", obj])
+    return obj
+
+
+def _c_obj(src, obj, includes):
+    inc = sum((["-I", d] for d in includes), [])
+    _run([CC] + C_FLAGS + inc + ["-c", src, "-o", obj])
+    return obj
+
+
+def build_library(jobs=8):
+    """libsrp_amd.so: host API + shader-independent kernels."""
+    os.makedirs(OBJ, exist_ok=True)
+    inc = [INCLUDE, os.path.join(CSRC, "lib"), os.path.join(CSRC, "host")]
+    hip_srcs = ["lib/runtime.cpp", "lib/assemble.hip", "lib/util.hip"]
+    c_srcs = ["host/math.c", "host/png_decode.c", "host/obj_load.c"]
+    tasks = []
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for s in hip_srcs:
+            o = os.path.join(OBJ, os.path.basename(s) + ".o")
+            tasks.append(ex.submit(_hip_obj, os.path.join(CSRC, s), o, inc))
+        for s in c_srcs:
+            o = os.path.join(OBJ, os.path.basename(s) + ".o")
+            tasks.append(ex.submit(_c_obj, os.path.join(CSRC, s), o, inc))
+        objs = [t.result() for t in tasks]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_shader_module(hip_sources, c_sources, out, includes=(), jobs=8):
+    """A shader module: HIP files with SRP_HIP_*_SHADER registrations plus
+    optional host C files, linked against libsrp_amd.so."""
+    objdir = os.path.join(os.path.dirname(out), "obj")
+    os.makedirs(objdir, exist_ok=True)
+    inc = [INCLUDE] + list(includes)
+    tasks = []
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for s in hip_sources:
+            o = os.path.join(objdir, os.path.basename(s) + ".o")
+            tasks.append(ex.submit(_hip_obj, s, o, inc))
+        for s in c_sources:
+            o = os.path.join(objdir, os.path.basename(s) + ".o")
+            tasks.append(ex.submit(_c_obj, s, o, inc))
+        objs = [t.result() for t in tasks]
+    rpath = os.path.relpath(PKG, os.path.dirname(out))
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L", PKG, "-lsrp_amd", f"-Wl,-rpath,$ORIGIN/{rpath}", "-lm"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_scenes(jobs=8):
+    """The GPU build of the test scene programs (tests/scenes)."""
+    host = os.path.join(CSRC, "host")
+    return build_shader_module(
+        [os.path.join(SCENE_DIR, "device_shaders.hip")],
+        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c")] + [os.path.join(host, "obj_load.c")],
+        SCENE_LIB, includes=[SCENE_DIR, host], jobs=jobs)
+
+
+def build_oracle():
+    """The CPU checkers (test infrastructure): the oracle and, when the
+    reference sources are present, the in-place reference build."""
+    _run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
+    if os.path.isdir("/root/reference/src"):
+        _run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"])
+
+
+def build_all(jobs=8):
+    build_library(jobs)
+    build_scenes(jobs)
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"all": build_all, "lib": build_library, "scenes": build_scenes, "oracle": build_oracle}[what]()
+    print("built", what)

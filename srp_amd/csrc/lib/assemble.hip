@@ -1,0 +1,774 @@
+/* Primitive assembly on the GPU: topology, index fetch, clipping, setup.
+ *
+ * One thread per input primitive, in two passes over the same code:
+ *   COUNT: records, clip-generated vertices and binning items (tile pairs
+ *          for filled triangles, fragments for lines and points) per input;
+ *   EMIT:  after exclusive scans of those counts, writes the records at
+ *          their final positions (so primitive IDs equal record indices,
+ *          matching the reference's sequential numbering after clipping
+ *          and culling, pipeline/primitive_assembly.c:87-120), materialises
+ *          the varyings of clip-generated vertices and writes the items.
+ *
+ * Arithmetic restates the reference (kitrofimov/srp) operation by operation:
+ * clipping pipeline/clipping.c:68-277, perspective divide
+ * pipeline/vertex_processing.c:76-88, triangle setup raster/triangle.c:113-238,
+ * line setup and DDA raster/line.c:34-86, points raster/point.c:32-118,
+ * NDC to screen core/framebuffer.c:48-55. */
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace srp {
+namespace lib {
+
+using srp::dev::roughlyZero;
+
+/* x86 cvttsd2si semantics: NaN and out-of-range give INT_MIN. */
+__device__ __forceinline__ int cvtI32(double d)
+{
+	if (!(d >= -2147483648.0 && d < 2147483648.0))
+		return (int) 0x80000000u;
+	return (int) d;
+}
+
+/* clipping.c:260-277 */
+__device__ __forceinline__ float planeDist(const float* c, int p)
+{
+	switch (p)
+	{
+		case 0: return c[0] + c[3];
+		case 1: return c[3] - c[0];
+		case 2: return c[1] + c[3];
+		case 3: return c[3] - c[1];
+		case 4: return c[2] + c[3];
+		default: return c[3] - c[2];
+	}
+}
+
+__device__ __forceinline__ uint32_t outcode(const float* c)
+{
+	uint32_t code = 0;
+#pragma unroll
+	for (int p = 0; p < 6; p++)
+		code |= (planeDist(c, p) < 0.0f) ? (1u << p) : 0u;
+	return code;
+}
+
+/* framebuffer.c:48-55 */
+__device__ __forceinline__ void toScreen(const SrpDrawParams* P, float nx, float ny, float& sx, float& sy)
+{
+	sx = (float) (((double) (float) P->width / 2.0) * ((double) nx + 1.0));
+	sy = (float) (((double) (float) P->height / 2.0) * (1.0 - (double) ny));
+}
+
+/* vertex_processing.c:76-88 */
+__device__ __forceinline__ float divide(float c[4])
+{
+	const float invW = (float) (1.0 / (double) c[3]);
+	c[0] *= invW;
+	c[1] *= invW;
+	c[2] *= invW;
+	c[3] = 1.0f;
+	return invW;
+}
+
+__device__ __forceinline__ uint64_t loadIndex(const uint8_t* ib, uint32_t elem, uint64_t i)
+{
+	switch (elem)
+	{
+		case 1: return ib[i];
+		case 2: return reinterpret_cast<const uint16_t*>(ib)[i];
+		case 4: return reinterpret_cast<const uint32_t*>(ib)[i];
+		default: return reinterpret_cast<const uint64_t*>(ib)[i];
+	}
+}
+
+__device__ __forceinline__ uint32_t slotOf(const AsmArgs& A, uint64_t stream)
+{
+	const uint64_t vi = A.indexElem ? loadIndex(A.indices, A.indexElem, stream) : stream;
+	return (uint32_t) (vi - A.P->cacheBase);
+}
+
+__device__ __forceinline__ void loadClip(const SrpDrawParams* P, uint32_t slot, float* c)
+{
+	const float4 v = reinterpret_cast<const float4*>(P->clip)[slot];
+	c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
+}
+
+/* ------------------------------------------------------------ clip pool */
+
+/* Vertices produced while clipping one primitive: the originals first, then
+ * each generated vertex as (parent a, parent b, t) with position
+ * a*(1-t) + b*t (clipping.c:244-258). */
+struct Pool
+{
+	float c[SRP_POOL][4];
+	uint8_t a[SRP_POOL], b[SRP_POOL];
+	float t[SRP_POOL];
+	uint32_t slot[3];
+	int n;
+};
+
+__device__ __forceinline__ int poolLerp(Pool& pl, int ia, int ib, float t)
+{
+	const int k = pl.n;
+	if (k >= SRP_POOL)
+		return -1;
+	for (int i = 0; i < 4; i++)
+		pl.c[k][i] = pl.c[ia][i] * (1 - t) + pl.c[ib][i] * t;
+	pl.a[k] = (uint8_t) ia;
+	pl.b[k] = (uint8_t) ib;
+	pl.t[k] = t;
+	pl.n = k + 1;
+	return k;
+}
+
+/* One Sutherland-Hodgman pass (clipping.c:199-242).  An edge that crosses
+ * the plane with |da-db| <= 1e-9 contributes nothing, not even its inside
+ * endpoint (clipping.c:226-227).  Returns the new count, -1 on overflow. */
+__device__ int clipPass(Pool& pl, const uint8_t* in, int n, int plane, uint8_t* out)
+{
+	int m = 0;
+	for (int i = 0; i < n; i++)
+	{
+		const int cur = in[i], nxt = in[(i + 1) % n];
+		const float da = planeDist(pl.c[cur], plane);
+		const float db = planeDist(pl.c[nxt], plane);
+		const bool ci = da >= 0.0f, ni = db >= 0.0f;
+		if (ci && ni)
+			out[m++] = (uint8_t) nxt;
+		else if (ci || ni)
+		{
+			if (roughlyZero(da - db))
+				continue;
+			const float t = da / (da - db);
+			const int k = poolLerp(pl, cur, nxt, t);
+			if (k < 0)
+				return -1;
+			out[m++] = (uint8_t) k;
+			if (!ci && ni)
+				out[m++] = (uint8_t) nxt;
+		}
+		if (m + 2 > SRP_MAX_POLY)
+			return -1;
+	}
+	return m;
+}
+
+/* clipping.c:68-119: the clipped polygon as pool indices; -1 on overflow. */
+__device__ int clipTriangle(Pool& pl, uint8_t* poly)
+{
+	const uint32_t c0 = outcode(pl.c[0]), c1 = outcode(pl.c[1]), c2 = outcode(pl.c[2]);
+	poly[0] = 0; poly[1] = 1; poly[2] = 2;
+	if ((c0 | c1 | c2) == 0)
+		return 3;
+	if ((c0 & c1 & c2) != 0)
+		return 0;
+	uint8_t tmp[SRP_MAX_POLY];
+	uint8_t* src = poly;
+	uint8_t* dst = tmp;
+	int n = 3;
+	for (int p = 0; p < 6; p++)
+	{
+		n = clipPass(pl, src, n, p, dst);
+		if (n <= 0)
+			return n;
+		uint8_t* s = src; src = dst; dst = s;
+	}
+	if (src != poly)
+		for (int i = 0; i < n; i++) poly[i] = src[i];
+	return n;
+}
+
+/* ----------------------------------------------------- generated varyings */
+
+/* Value of one float element for pool vertex k under the clip rule
+ * (interpolation with invW == NULL: every float varying is blended
+ * affinely, interpolation.c:116-120, weights {1-t, t}).  Parents always
+ * precede their children in the pool, so one forward sweep suffices. */
+__device__ float genFloat(const SrpDrawParams* P, const Pool& pl, int k, uint32_t off)
+{
+	float val[SRP_POOL];
+	for (int q = 0; q <= k; q++)
+	{
+		if (q < 3)
+			val[q] = srp::dev::loadF(srp::dev::varyingPtr(P, pl.slot[q]) + off);
+		else
+		{
+			const float t = pl.t[q];
+			const float w0 = 1 - t;
+			float v = 0.0f;
+			v += val[pl.a[q]] * w0;
+			v += val[pl.b[q]] * t;
+			val[q] = v;
+		}
+	}
+	return val[k];
+}
+
+__device__ double genDouble(const SrpDrawParams* P, const Pool& pl, int k, uint32_t off)
+{
+	double val[SRP_POOL];
+	for (int q = 0; q <= k; q++)
+	{
+		if (q < 3)
+			val[q] = srp::dev::loadD(srp::dev::varyingPtr(P, pl.slot[q]) + off);
+		else
+		{
+			const float t = pl.t[q];
+			const float w0 = 1 - t;
+			double v = 0.0;
+			v += val[pl.a[q]] * (double) w0;
+			v += val[pl.b[q]] * (double) t;
+			val[q] = v;
+		}
+	}
+	return val[k];
+}
+
+/* Integers come from the pair's provoking vertex: b for LAST, a for FIRST. */
+__device__ __forceinline__ int genIntSource(const SrpDrawParams* P, const Pool& pl, int k)
+{
+	while (k >= 3)
+		k = P->st.provokingLast ? pl.b[k] : pl.a[k];
+	return k;
+}
+
+__device__ void materialize(const SrpDrawParams* P, const Pool& pl, int k, uint8_t* dst)
+{
+	for (uint32_t a = 0; a < P->nVaryings; a++)
+	{
+		const SrpVaryingDesc d = P->var[a];
+		if (d.type == SRP_FLOAT)
+			for (uint32_t e = 0; e < d.nItems; e++)
+			{
+				const float v = genFloat(P, pl, k, d.offset + 4 * e);
+				__builtin_memcpy(dst + d.offset + 4 * e, &v, 4);
+			}
+		else if (d.type == SRP_DOUBLE)
+			for (uint32_t e = 0; e < d.nItems; e++)
+			{
+				const double v = genDouble(P, pl, k, d.offset + 8 * e);
+				__builtin_memcpy(dst + d.offset + 8 * e, &v, 8);
+			}
+		else
+		{
+			const uint32_t n = srp::dev::elemSize(d.type) * d.nItems;
+			const uint8_t* src = srp::dev::varyingPtr(P, pl.slot[genIntSource(P, pl, k)]) + d.offset;
+			for (uint32_t b = 0; b < n; b++) dst[d.offset + b] = src[b];
+		}
+	}
+}
+
+/* ------------------------------------------------------------------ setup */
+
+struct Vtx
+{
+	float c[4];      /* ndc after the divide */
+	float invW;
+	uint32_t vref;
+};
+
+__device__ __forceinline__ float minf_(float a, float b) { return a > b ? b : a; }
+__device__ __forceinline__ float maxf_(float a, float b) { return a > b ? a : b; }
+
+/* triangle.c:113-238.  Returns false when culled or degenerate. */
+__device__ bool setupTriangle(const SrpDrawParams* P, Vtx v0, Vtx v1, Vtx v2, uint32_t id, SrpTri& T)
+{
+	const SrpState& st = P->st;
+	const float e0x = v1.c[0] - v0.c[0], e0y = v1.c[1] - v0.c[1];
+	const float e1x = v2.c[0] - v0.c[0], e1y = v2.c[1] - v0.c[1];
+	const float area = e0x * e1y - e0y * e1x;
+	const bool ccw = area > 0.0f;
+	if (st.cullFace == SRP_FACE_FRONT_AND_BACK)
+		return false;
+	const bool front = ((area > 0.0f) && st.frontFace == SRP_WINDING_CCW) || ((area < 0.0f) && st.frontFace == SRP_WINDING_CW);
+	const bool cull = (front && st.cullFace == SRP_FACE_FRONT) || (!front && st.cullFace == SRP_FACE_BACK);
+	if (cull)
+		return false;
+	if (!ccw)
+	{
+		const Vtx t = v1; v1 = v2; v2 = t;
+	}
+	const Vtx v[3] = {v0, v1, v2};
+	float sx[3], sy[3];
+	for (int i = 0; i < 3; i++) toScreen(P, v[i].c[0], v[i].c[1], sx[i], sy[i]);
+	float ex[3], ey[3];
+	for (int i = 0; i < 3; i++)
+	{
+		ex[i] = sx[(i + 1) % 3] - sx[i];
+		ey[i] = sy[(i + 1) % 3] - sy[i];
+	}
+	const float areaX2 = fabsf(ex[0] * ey[2] - ey[0] * ex[2]);
+	if (roughlyZero(areaX2))
+		return false;
+
+	const float mnx = minf_(sx[0], minf_(sx[1], sx[2])), mny = minf_(sy[0], minf_(sy[1], sy[2]));
+	const float mxx = maxf_(sx[0], maxf_(sx[1], sx[2])), mxy = maxf_(sy[0], maxf_(sy[1], sy[2]));
+	const double fx = floor((double) mnx), fy = floor((double) mny);
+	const double cx = ceil((double) mxx), cy = ceil((double) mxy);
+	const float minX = (float) (fx > 0.0 ? fx : 0.0);
+	const float minY = (float) (fy > 0.0 ? fy : 0.0);
+	const float maxX = (float) (cx > (double) P->width ? (double) P->width : cx);
+	const float maxY = (float) (cy > (double) P->height ? (double) P->height : cy);
+
+	const float px = (float) ((double) minX + 0.5), py = (float) ((double) minY + 0.5);
+	for (int i = 0; i < 3; i++)
+	{
+		const int j = (i + 1) % 3;
+		const float rx = px - sx[j], ry = py - sy[j];
+		T.l0[i] = (rx * ey[j] - ry * ex[j]) / areaX2;
+		T.dx[i] = ey[j] / areaX2;
+		T.dy[i] = -ex[j] / areaX2;
+	}
+	uint32_t flags = front ? 8u : 0u;
+	for (int i = 0; i < 3; i++)
+		if (((ex[i] > 0.0f) && roughlyZero(ey[i])) || (ey[i] < 0.0f))
+			flags |= 1u << i;
+	/* The reference's scan runs while (float) x < maxX from (size_t) minX;
+	 * a NaN or inverted box scans nothing. */
+	const int x0 = (int) minX, y0 = (int) minY;
+	const int x1 = (maxX > minX) ? (int) maxX : x0;
+	const int y1 = (maxY > minY) ? (int) maxY : y0;
+	T.x0 = (uint16_t) x0; T.y0 = (uint16_t) y0;
+	T.x1 = (uint16_t) x1; T.y1 = (uint16_t) y1;
+	T.flags = flags;
+	T.id = id;
+	for (int i = 0; i < 3; i++)
+	{
+		T.invW[i] = v[i].invW;
+		T.zw[i] = v[i].c[2] * v[i].invW;
+		T.vref[i] = v[i].vref;
+	}
+	T.pad = 0;
+	return true;
+}
+
+__device__ __forceinline__ uint32_t tilePairs(const SrpDrawParams* P, const SrpTri& T)
+{
+	if (T.x1 <= T.x0 || T.y1 <= T.y0)
+		return 0;
+	const uint32_t s = P->tileShift;
+	return (((T.x1 - 1u) >> s) - (T.x0 >> s) + 1u) * (((T.y1 - 1u) >> s) - (T.y0 >> s) + 1u);
+}
+
+/* line.c:79-86 */
+__device__ void setupLine(const SrpDrawParams* P, const Vtx& a, const Vtx& b, uint32_t id, SrpLine& L)
+{
+	toScreen(P, a.c[0], a.c[1], L.ssx[0], L.ssy[0]);
+	toScreen(P, b.c[0], b.c[1], L.ssx[1], L.ssy[1]);
+	L.invW[0] = a.invW; L.invW[1] = b.invW;
+	L.zw[0] = a.c[2] * a.invW; L.zw[1] = b.c[2] * b.invW;
+	L.vref[0] = a.vref; L.vref[1] = b.vref;
+	L.id = id;
+	L.pad = 0;
+}
+
+/* line.c:41-46: number of DDA samples (steps + 1); 0 when steps is not finite. */
+__device__ __forceinline__ int lineSteps(const SrpLine& L)
+{
+	const float dx = L.ssx[1] - L.ssx[0], dy = L.ssy[1] - L.ssy[0];
+	int steps = cvtI32(ceil(fmax(fabs((double) dx), fabs((double) dy))));
+	if (steps == 0) steps = 1;
+	return steps;
+}
+
+__device__ __forceinline__ uint32_t lineFragCount(const SrpLine& L)
+{
+	const int steps = lineSteps(L);
+	return steps >= 0 ? (uint32_t) steps + 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t pixelKey(const SrpDrawParams* P, int px, int py)
+{
+	if (px < 0 || py < 0)
+		return 0xFFFFFFFFu;
+	const uint64_t idx = (uint64_t) py * P->width + (uint64_t) px;
+	return idx < (uint64_t) P->width * P->height ? (uint32_t) idx : 0xFFFFFFFFu;
+}
+
+/* line.c:48-76: the DDA walks x, y, t by repeated float addition. */
+__device__ void emitLineFrags(const AsmArgs& A, const SrpLine& L, uint32_t rec, uint32_t base)
+{
+	const int steps = lineSteps(L);
+	if (steps < 0) return;
+	const float dx = L.ssx[1] - L.ssx[0], dy = L.ssy[1] - L.ssy[0];
+	const float xInc = dx / (float) steps, yInc = dy / (float) steps;
+	const float tInc = (float) (1.0 / (double) steps);
+	float x = L.ssx[0], y = L.ssy[0], t = 0.0f;
+	uint32_t invalid = 0;
+	for (int i = 0; i <= steps; i++)
+	{
+		const int px = cvtI32(round((double) x)), py = cvtI32(round((double) y));
+		SrpFrag f;
+		f.px = px; f.py = py; f.prim = rec; f.t = t;
+		const uint32_t key = pixelKey(A.P, px, py);
+		invalid += key == 0xFFFFFFFFu;
+		A.frags[base + i] = f;
+		A.fragKeys[base + i] = key;
+		x += xInc; y += yInc; t += tInc;
+	}
+	if (invalid)
+		atomicAdd(&A.P->stats[0], (unsigned long long) invalid);
+}
+
+/* point.c:81-118: pixel bounds of a point; false when fully outside. */
+struct PointBox { float mnx, mny, mxx, mxy; int x0, x1, y0, y1; };
+
+__device__ bool pointBox(const SrpDrawParams* P, const SrpPoint& pt, PointBox& b)
+{
+	const float half = (float) ((double) P->st.pointSize * 0.5);
+	b.mnx = pt.ssx - half; b.mny = pt.ssy - half;
+	b.mxx = pt.ssx + half; b.mxy = pt.ssy + half;
+	int x0 = cvtI32(floor((double) b.mnx)), x1 = cvtI32(floor((double) b.mxx));
+	int y0 = cvtI32(floor((double) b.mny)), y1 = cvtI32(floor((double) b.mxy));
+	if (x1 < 0 || y1 < 0 || x0 >= (int) P->width || y0 >= (int) P->height)
+		return false;
+	if (x0 < 0) x0 = 0;
+	if (y0 < 0) y0 = 0;
+	if (x1 >= (int) P->width) x1 = (int) P->width - 1;
+	if (y1 >= (int) P->height) y1 = (int) P->height - 1;
+	b.x0 = x0; b.x1 = x1; b.y0 = y0; b.y1 = y1;
+	return true;
+}
+
+__device__ __forceinline__ bool pointCovers(const PointBox& b, int x, int y)
+{
+	const float px = (float) ((double) x + 0.5), py = (float) ((double) y + 0.5);
+	return !(px < b.mnx || px >= b.mxx || py < b.mny || py >= b.mxy);
+}
+
+template <bool EMIT>
+__device__ uint32_t pointFrags(const AsmArgs& A, const SrpPoint& pt, uint32_t rec, uint32_t base)
+{
+	PointBox b;
+	if (!pointBox(A.P, pt, b))
+		return 0;
+	uint32_t n = 0;
+	for (int y = b.y0; y <= b.y1; y++)
+		for (int x = b.x0; x <= b.x1; x++)
+			if (pointCovers(b, x, y))
+			{
+				if (EMIT)
+				{
+					SrpFrag f;
+					f.px = x; f.py = y; f.prim = rec; f.t = 0.0f;
+					A.frags[base + n] = f;
+					A.fragKeys[base + n] = (uint32_t) y * A.P->width + (uint32_t) x;
+				}
+				n++;
+			}
+	return n;
+}
+
+__device__ __forceinline__ void makePoint(const SrpDrawParams* P, const Vtx& v, uint32_t id, SrpPoint& pt)
+{
+	toScreen(P, v.c[0], v.c[1], pt.ssx, pt.ssy);
+	pt.z = v.c[2];
+	pt.w = v.c[3];
+	pt.vref = v.vref;
+	pt.id = id;
+	pt.pad[0] = pt.pad[1] = 0;
+}
+
+/* ---------------------------------------------------------------- kernels */
+
+/* topology.c:24-48 */
+__device__ __forceinline__ void triStream(uint32_t prim, uint64_t base, uint64_t k, uint64_t* s)
+{
+	if (prim == SRP_PRIM_TRIANGLES) { s[0] = base + 3 * k; s[1] = base + 3 * k + 1; s[2] = base + 3 * k + 2; }
+	else if (prim == SRP_PRIM_TRIANGLE_STRIP)
+	{
+		const bool odd = (k & 1) != 0;
+		s[0] = base + k + (odd ? 1 : 0);
+		s[1] = base + k + (odd ? 0 : 1);
+		s[2] = base + k + 2;
+	}
+	else { s[0] = base; s[1] = base + k + 1; s[2] = base + k + 2; }
+}
+
+/* Triangle draws (all polygon modes): primitive_assembly.c:37-135. */
+template <bool EMIT>
+__global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
+{
+	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
+	if (k >= A.nIn)
+		return;
+	const SrpDrawParams* P = A.P;
+	Pool pl;
+	uint64_t s[3];
+	triStream(A.prim, A.start, k, s);
+	for (int i = 0; i < 3; i++)
+	{
+		pl.slot[i] = slotOf(A, s[i]);
+		loadClip(P, pl.slot[i], pl.c[i]);
+	}
+	pl.n = 3;
+	uint8_t poly[SRP_MAX_POLY];
+	int n = clipTriangle(pl, poly);
+	if (n < 0)
+	{
+		atomicOr(A.error, 1u);
+		n = 0;
+	}
+
+	uint32_t recBase = 0, genBase = 0, itemBase = 0;
+	if (EMIT)
+	{
+		recBase = A.recOff[k];
+		genBase = A.genOff[k];
+		itemBase = A.itemOff[k];
+	}
+
+	/* Divide each polygon vertex once; number the generated ones. */
+	Vtx v[SRP_MAX_POLY];
+	uint32_t nGen = 0;
+	for (int i = 0; i < n; i++)
+	{
+		const int q = poly[i];
+		for (int c = 0; c < 4; c++) v[i].c[c] = pl.c[q][c];
+		v[i].invW = divide(v[i].c);
+		if (q < 3)
+			v[i].vref = pl.slot[q];
+		else
+		{
+			v[i].vref = P->cacheSize + genBase + nGen;
+			if (EMIT)
+				materialize(P, pl, q, A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
+			nGen++;
+		}
+	}
+
+	uint32_t nRec = 0, nItems = 0;
+	const uint32_t mode = P->st.polygonMode;
+	for (int f = 1; f + 1 < n; f++)
+	{
+		const Vtx& a = v[0];
+		const Vtx& b = v[f];
+		const Vtx& c = v[f + 1];
+		if (mode == SRP_POLYGON_MODE_FILL)
+		{
+			SrpTri T;
+			if (!setupTriangle(P, a, b, c, recBase + nRec, T))
+				continue;
+			const uint32_t np = tilePairs(P, T);
+			if (EMIT)
+			{
+				const uint32_t r = recBase + nRec;
+				A.tris[r] = T;
+				if (np)
+				{
+					const uint32_t sh = P->tileShift;
+					uint32_t o = itemBase + nItems;
+					for (uint32_t ty = T.y0 >> sh; ty <= (T.y1 - 1u) >> sh; ty++)
+						for (uint32_t tx = T.x0 >> sh; tx <= (T.x1 - 1u) >> sh; tx++)
+						{
+							const uint32_t key = ty * P->tilesX + tx;
+							A.pairKeys[o] = key;
+							A.pairVals[o] = r;
+							atomicAdd(&A.tileCount[key], 1u);
+							o++;
+						}
+				}
+			}
+			nRec++;
+			nItems += np;
+		}
+		else if (mode == SRP_POLYGON_MODE_LINE)
+		{
+			const Vtx* tv[3] = {&a, &b, &c};
+			for (int j = 0; j < 3; j++)
+			{
+				SrpLine L;
+				setupLine(P, *tv[j], *tv[(j + 1) % 3], recBase + nRec, L);
+				const uint32_t nf = lineFragCount(L);
+				if (EMIT)
+				{
+					A.lines[recBase + nRec] = L;
+					emitLineFrags(A, L, recBase + nRec, itemBase + nItems);
+				}
+				nRec++;
+				nItems += nf;
+			}
+		}
+		else
+		{
+			const Vtx* tv[3] = {&a, &b, &c};
+			for (int j = 0; j < 3; j++)
+			{
+				SrpPoint pt;
+				makePoint(P, *tv[j], recBase + nRec, pt);
+				if (EMIT) A.points[recBase + nRec] = pt;
+				nItems += pointFrags<EMIT>(A, pt, recBase + nRec, itemBase + nItems);
+				nRec++;
+			}
+		}
+	}
+	if (!EMIT)
+	{
+		A.recOff[k] = nRec;
+		A.genOff[k] = nGen;
+		A.itemOff[k] = nItems;
+	}
+}
+
+/* topology.c:62-83 */
+__device__ __forceinline__ void lineStream(uint32_t prim, uint64_t base, uint64_t k, uint64_t count, uint64_t* s)
+{
+	if (prim == SRP_PRIM_LINES) { s[0] = base + 2 * k; s[1] = base + 2 * k + 1; }
+	else if (prim == SRP_PRIM_LINE_STRIP) { s[0] = base + k; s[1] = base + k + 1; }
+	else { s[0] = base + k; s[1] = base + (k + 1) % count; }
+}
+
+/* Line draws: primitive_assembly.c:137-178 with clipping.c:139-183. */
+template <bool EMIT>
+__global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
+{
+	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
+	if (k >= A.nIn)
+		return;
+	const SrpDrawParams* P = A.P;
+	Pool pl;
+	uint64_t s[2];
+	lineStream(A.prim, A.start, k, A.count, s);
+	for (int i = 0; i < 2; i++)
+	{
+		pl.slot[i] = slotOf(A, s[i]);
+		loadClip(P, pl.slot[i], pl.c[i]);
+	}
+	pl.slot[2] = pl.slot[1];
+	pl.n = 2;
+
+	/* Liang-Barsky style clip: both endpoints re-derived from the originals. */
+	int ia = 0, ib = 1;
+	bool rejected = false;
+	const uint32_t c0 = outcode(pl.c[0]), c1 = outcode(pl.c[1]);
+	if ((c0 | c1) != 0)
+	{
+		if ((c0 & c1) != 0)
+			rejected = true;
+		else
+		{
+			float t0 = 0.0f, t1 = 1.0f;
+			for (int p = 0; p < 6 && !rejected; p++)
+			{
+				const float da = planeDist(pl.c[0], p), db = planeDist(pl.c[1], p);
+				if (da < 0.0f && db < 0.0f) { rejected = true; break; }
+				if (da < 0.0f || db < 0.0f)
+				{
+					if (roughlyZero(da - db)) continue;
+					const float t = da / (da - db);
+					if (da < 0.0f) t0 = (t0 > t) ? t0 : t;
+					else t1 = (t1 > t) ? t : t1;
+					if (t0 > t1) rejected = true;
+				}
+			}
+			if (!rejected)
+			{
+				/* pool slot 2 is unused for lines; generated vertices start at 3 */
+				pl.n = 3;
+				if (t0 > 0.0f) ia = poolLerp(pl, 0, 1, t0);
+				if (t1 < 1.0f) ib = poolLerp(pl, 0, 1, t1);
+			}
+		}
+	}
+
+	uint32_t nRec = 0, nGen = 0, nItems = 0;
+	if (!rejected)
+	{
+		const uint32_t recBase = EMIT ? A.recOff[k] : 0u;
+		const uint32_t genBase = EMIT ? A.genOff[k] : 0u;
+		const uint32_t itemBase = EMIT ? A.itemOff[k] : 0u;
+		Vtx e[2];
+		const int q[2] = {ia, ib};
+		for (int i = 0; i < 2; i++)
+		{
+			for (int c = 0; c < 4; c++) e[i].c[c] = pl.c[q[i]][c];
+			e[i].invW = divide(e[i].c);
+			if (q[i] < 3)
+				e[i].vref = pl.slot[q[i]];
+			else
+			{
+				e[i].vref = P->cacheSize + genBase + nGen;
+				if (EMIT)
+					materialize(P, pl, q[i], A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
+				nGen++;
+			}
+		}
+		SrpLine L;
+		setupLine(P, e[0], e[1], recBase, L);
+		nItems = lineFragCount(L);
+		if (EMIT)
+		{
+			A.lines[recBase] = L;
+			emitLineFrags(A, L, recBase, itemBase);
+		}
+		nRec = 1;
+	}
+	if (!EMIT)
+	{
+		A.recOff[k] = nRec;
+		A.genOff[k] = nGen;
+		A.itemOff[k] = nItems;
+	}
+}
+
+/* Point draws: primitive_assembly.c:221-254, clipping.c:185-197. */
+template <bool EMIT>
+__global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
+{
+	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
+	if (k >= A.nIn)
+		return;
+	const SrpDrawParams* P = A.P;
+	const uint32_t slot = slotOf(A, A.start + k);
+	float c[4];
+	loadClip(P, slot, c);
+	const float x = c[0], y = c[1], z = c[2], w = c[3];
+	const bool clipped = (x < -w || x > w) || (y < -w || y > w) || (z < -w || z > w);
+	uint32_t nRec = 0, nItems = 0;
+	if (!clipped)
+	{
+		const uint32_t recBase = EMIT ? A.recOff[k] : 0u;
+		const uint32_t itemBase = EMIT ? A.itemOff[k] : 0u;
+		Vtx v;
+		for (int i = 0; i < 4; i++) v.c[i] = c[i];
+		v.invW = divide(v.c);
+		v.vref = slot;
+		SrpPoint pt;
+		makePoint(P, v, recBase, pt);
+		if (EMIT) A.points[recBase] = pt;
+		nItems = pointFrags<EMIT>(A, pt, recBase, itemBase);
+		nRec = 1;
+	}
+	if (!EMIT)
+	{
+		A.recOff[k] = nRec;
+		A.genOff[k] = 0;
+		A.itemOff[k] = nItems;
+	}
+}
+
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s)
+{
+	const uint32_t blocks = (A.nIn + 127u) / 128u;
+	if (!blocks) return;
+	switch (kind)
+	{
+		case 0:
+			if (emit) hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);
+			else hipLaunchKernelGGL(k_assemble_tris<false>, dim3(blocks), dim3(128), 0, s, A);
+			break;
+		case 1:
+			if (emit) hipLaunchKernelGGL(k_assemble_lines<true>, dim3(blocks), dim3(128), 0, s, A);
+			else hipLaunchKernelGGL(k_assemble_lines<false>, dim3(blocks), dim3(128), 0, s, A);
+			break;
+		default:
+			if (emit) hipLaunchKernelGGL(k_assemble_points<true>, dim3(blocks), dim3(128), 0, s, A);
+			else hipLaunchKernelGGL(k_assemble_points<false>, dim3(blocks), dim3(128), 0, s, A);
+			break;
+	}
+}
+
+}  // namespace lib
+}  // namespace srp

@@ -1,0 +1,75 @@
+/* Library-internal kernels and their host launchers. */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "srp/buffer.h"
+#include "srp/device/abi.h"
+#include "srp/device/pipeline.hpp"
+
+#define SRP_POOL 32       /* clip pool: 3 originals + generated vertices */
+#define SRP_MAX_POLY 16   /* vertices of a clipped polygon */
+
+namespace srp {
+namespace lib {
+
+/* Arguments of the assembly kernels (assemble.hip). */
+struct AsmArgs
+{
+	const SrpDrawParams* P;
+	uint32_t prim;        /* SRPPrimitive */
+	uint32_t nIn;         /* input primitives */
+	uint64_t start;       /* first stream index */
+	uint64_t count;       /* stream indices of the draw (line loops wrap) */
+	const uint8_t* indices;
+	uint32_t indexElem;   /* 0: non-indexed, else bytes per index */
+	/* COUNT pass: per-input counts; EMIT pass: their exclusive scans */
+	uint32_t* recOff;
+	uint32_t* genOff;
+	uint32_t* itemOff;
+	/* EMIT outputs */
+	SrpTri* tris;
+	SrpLine* lines;
+	SrpPoint* points;
+	uint32_t* pairKeys;
+	uint32_t* pairVals;
+	SrpFrag* frags;
+	uint32_t* fragKeys;
+	uint32_t* tileCount;
+	uint8_t* genVaryings;
+	uint32_t* error;
+};
+
+/* kind: 0 triangles, 1 lines, 2 points */
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s);
+
+/* util.hip */
+struct ScanScratch
+{
+	uint32_t* buf = nullptr;   /* block sums of every level */
+	size_t cap = 0;
+};
+/* Exclusive prefix sum of n u32 values in place; *total (device) = sum. */
+void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scratch, hipStream_t s);
+
+struct SortScratch
+{
+	uint32_t* keys = nullptr;
+	uint32_t* vals = nullptr;
+	uint32_t* hist = nullptr;
+	size_t cap = 0, histCap = 0;
+	ScanScratch scan;
+};
+/* Stable LSD radix sort of (keys, vals) on the low `bits` bits of the keys.
+ * The sorted arrays are returned through *outKeys / *outVals (either the
+ * inputs or the scratch buffers). */
+void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& scratch,
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s);
+
+void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
+void iota(uint32_t* p, size_t n, hipStream_t s);
+
+}  // namespace lib
+}  // namespace srp

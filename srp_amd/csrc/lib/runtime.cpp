@@ -1,0 +1,1145 @@
+/* Host side of libsrp_amd: the srp C API (include/srp) over the HIP
+ * pipeline.  Entry points keep the reference's names, argument meaning and
+ * error behaviour (message callback, early return; kitrofimov/srp
+ * core/*.c, pipeline/draw.c); the work of a draw runs on the GPU:
+ *
+ *   vertex shader (shader module)  ->  assembly/clip/setup COUNT pass
+ *   -> scans -> one host sync for the totals -> EMIT pass
+ *   -> filled triangles: stable radix sort of (tile, primitive) pairs,
+ *      per-tile rasteriser (shader module)
+ *   -> lines/points: stable radix sort of fragments by pixel, per-pixel
+ *      ordered fragment kernel (shader module). */
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#define SRP_INCLUDE_VEC
+#define SRP_INCLUDE_MAT
+#include "srp/srp.h"
+#include "srp/srp_hip.h"
+#include "srp/device/abi.h"
+#include "kernels.h"
+#include "../host/png_decode.h"
+
+using namespace srp::lib;
+
+/* The user program defines srpContext (reference include/srp/context.h:194);
+ * this weak definition only serves programs that never do. */
+extern "C" {
+__attribute__((weak)) SRPContext srpContext;
+}
+
+namespace {
+
+#define HIP_CHECK(x)                                                                        \
+	do {                                                                                    \
+		hipError_t e_ = (x);                                                                \
+		if (e_ != hipSuccess)                                                               \
+		{                                                                                   \
+			fprintf(stderr, "libsrp_amd: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
+			        __FILE__, __LINE__);                                                    \
+			abort();                                                                        \
+		}                                                                                   \
+	} while (0)
+
+SRPContext* g_ctx = nullptr;
+
+SRPContext* ctx() { return g_ctx ? g_ctx : &srpContext; }
+
+/* utils/message_callback.c:18-35 */
+void message(SRPMessageType type, SRPMessageSeverity sev, const char* fn, const char* fmt, ...)
+{
+	SRPContext* c = ctx();
+	if (c->messageCallback.func == nullptr)
+		return;
+	char text[1024];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(text, sizeof(text), fmt, ap);
+	va_end(ap);
+	c->messageCallback.func(type, sev, fn, text, c->messageCallback.userParameter);
+}
+
+/* Grow-only device allocation. */
+struct DevBuf
+{
+	void* p = nullptr;
+	size_t cap = 0;
+	template <typename T>
+	T* get(size_t bytes)
+	{
+		if (bytes > cap)
+		{
+			/* in-flight kernels of earlier draws may still use the old block */
+			if (p)
+			{
+				HIP_CHECK(hipDeviceSynchronize());
+				HIP_CHECK(hipFree(p));
+			}
+			cap = bytes + bytes / 4 + 256;
+			HIP_CHECK(hipMalloc(&p, cap));
+		}
+		return static_cast<T*>(p);
+	}
+};
+
+struct Runtime
+{
+	hipStream_t stream = nullptr;
+	SrpDrawParams* hParams = nullptr;   /* pinned staging */
+	SrpDrawParams* dParams = nullptr;
+	uint8_t* hUniform = nullptr;        /* pinned staging */
+	size_t uniformCap = 0;
+	uint32_t* hTotals = nullptr;        /* pinned: rec, gen, items, error */
+	unsigned long long* dStats = nullptr;
+	uint64_t hostPrims = 0, hostDraws = 0;
+	int tile = 8;
+
+	DevBuf uniform, clip, varyings, genVaryings, counts, totals;
+	DevBuf tris, lines, points, pairKeys, pairVals, frags, fragKeys, order, tileCount;
+	ScanScratch scan;
+	SortScratch sort;
+
+	std::unordered_map<const void*, const SrpVsEntry*> vs;
+	std::unordered_map<const void*, const SrpFsEntry*> fs;
+
+	bool profile = false;
+	double profMs[4] = {0, 0, 0, 0};
+	uint64_t profLaunches[4] = {0, 0, 0, 0};
+	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+	hipEvent_t paramsCopied = nullptr;  /* staging block free again */
+
+	Runtime()
+	{
+		HIP_CHECK(hipSetDevice(0));
+		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+		HIP_CHECK(hipHostMalloc((void**) &hParams, sizeof(SrpDrawParams), hipHostMallocDefault));
+		HIP_CHECK(hipMalloc((void**) &dParams, sizeof(SrpDrawParams)));
+		HIP_CHECK(hipHostMalloc((void**) &hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
+		HIP_CHECK(hipMalloc((void**) &dStats, 4 * sizeof(unsigned long long)));
+		HIP_CHECK(hipMemset(dStats, 0, 4 * sizeof(unsigned long long)));
+		for (int i = 0; i < 4; i++) HIP_CHECK(hipEventCreate(&ev[i]));
+		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
+		const char* t = getenv("SRP_TILE");
+		if (t && (atoi(t) == 8 || atoi(t) == 16)) tile = atoi(t);
+	}
+};
+
+Runtime& rt()
+{
+	static Runtime* r = new Runtime();
+	return *r;
+}
+
+/* Registries live outside the runtime so that shader modules loaded before
+ * the first API call register without touching the GPU. */
+std::mutex g_regLock;
+std::unordered_map<const void*, const SrpVsEntry*>& vsRegistry()
+{
+	static auto* m = new std::unordered_map<const void*, const SrpVsEntry*>();
+	return *m;
+}
+std::unordered_map<const void*, const SrpFsEntry*>& fsRegistry()
+{
+	static auto* m = new std::unordered_map<const void*, const SrpFsEntry*>();
+	return *m;
+}
+
+/* ---------------------------------------------------------------- objects */
+
+struct FbImpl
+{
+	SRPFramebuffer pub;       /* first member: SRPFramebuffer* == FbImpl* */
+	uint32_t* dColor = nullptr;
+	float* dDepth = nullptr;
+	uint8_t* dStencil = nullptr;
+	bool deferred = false;
+};
+
+FbImpl* fbImpl(const SRPFramebuffer* fb) { return (FbImpl*) fb; }
+
+void fbUpload(FbImpl* F)
+{
+	Runtime& R = rt();
+	const size_t n = F->pub.size;
+	HIP_CHECK(hipMemcpyAsync(F->dColor, F->pub.color, n * 4, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(F->dDepth, F->pub.depth, n * 4, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(F->dStencil, F->pub.stencil, n, hipMemcpyHostToDevice, R.stream));
+}
+
+void fbDownload(FbImpl* F)
+{
+	Runtime& R = rt();
+	const size_t n = F->pub.size;
+	HIP_CHECK(hipMemcpyAsync(F->pub.color, F->dColor, n * 4, hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(F->pub.depth, F->dDepth, n * 4, hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(F->pub.stencil, F->dStencil, n, hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+}
+
+struct TexImpl
+{
+	SrpTexDesc* dDesc = nullptr;
+	uint32_t* dTexels = nullptr;
+	std::vector<uint8_t> rgb;
+	int w = 0, h = 0;
+	SrpTexDesc desc;
+};
+
+std::unordered_map<const void*, TexImpl*>& textures()
+{
+	static auto* m = new std::unordered_map<const void*, TexImpl*>();
+	return *m;
+}
+
+TexImpl* texImpl(const SRPTexture* t)
+{
+	auto& m = textures();
+	auto it = m.find((const void*) t);
+	return it == m.end() ? nullptr : it->second;
+}
+
+SRPTexture* makeTexture(uint8_t* rgb, int w, int h, SRPTextureWrappingMode wx, SRPTextureWrappingMode wy)
+{
+	Runtime& R = rt();
+	TexImpl* T = new TexImpl();
+	T->w = w;
+	T->h = h;
+	T->rgb.assign(rgb, rgb + (size_t) w * h * 3);
+	std::vector<uint32_t> rgba((size_t) w * h);
+	for (size_t i = 0; i < rgba.size(); i++)
+		rgba[i] = (uint32_t) rgb[3 * i] | ((uint32_t) rgb[3 * i + 1] << 8) | ((uint32_t) rgb[3 * i + 2] << 16) | 0xFF000000u;
+	HIP_CHECK(hipMalloc((void**) &T->dTexels, rgba.size() * 4 + 16));
+	HIP_CHECK(hipMalloc((void**) &T->dDesc, sizeof(SrpTexDesc)));
+	T->desc.texels = T->dTexels;
+	T->desc.w = w; T->desc.h = h;
+	T->desc.wM1 = w - 1; T->desc.hM1 = h - 1;
+	T->desc.wrapX = wx; T->desc.wrapY = wy;
+	HIP_CHECK(hipMemcpyAsync(T->dTexels, rgba.data(), rgba.size() * 4, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(T->dDesc, &T->desc, sizeof(SrpTexDesc), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	textures()[(const void*) T->dDesc] = T;
+	return (SRPTexture*) T->dDesc;
+}
+
+void texPush(TexImpl* T)
+{
+	Runtime& R = rt();
+	HIP_CHECK(hipMemcpyAsync(T->dDesc, &T->desc, sizeof(SrpTexDesc), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+}
+
+}  // namespace
+
+struct SRPVertexBuffer
+{
+	size_t nBytesPerVertex = 0, nVertices = 0, nBytesAllocated = 0;
+	uint8_t* dData = nullptr;
+};
+
+struct SRPIndexBuffer
+{
+	SRPType type = SRP_UINT8;
+	size_t elem = 1, nIndices = 0, nBytesAllocated = 0;
+	uint8_t* dData = nullptr;
+	std::vector<uint8_t> host;
+	/* min/max vertex index per block of 4096 indices, for range queries */
+	std::vector<uint64_t> blkMin, blkMax;
+};
+
+namespace {
+
+constexpr size_t kIdxBlock = 4096;
+
+uint64_t hostIndex(const SRPIndexBuffer* b, size_t i)
+{
+	const uint8_t* p = b->host.data() + i * b->elem;
+	switch (b->elem)
+	{
+		case 1: return *p;
+		case 2: { uint16_t v; memcpy(&v, p, 2); return v; }
+		case 4: { uint32_t v; memcpy(&v, p, 4); return v; }
+		default: { uint64_t v; memcpy(&v, p, 8); return v; }
+	}
+}
+
+/* vertex_processing.c:90-112: smallest and largest vertex index of the draw. */
+void indexRange(const SRPIndexBuffer* b, size_t start, size_t count, uint64_t& lo, uint64_t& hi)
+{
+	lo = UINT64_MAX;
+	hi = 0;
+	size_t i = start;
+	const size_t end = start + count;
+	while (i < end)
+	{
+		if (i % kIdxBlock == 0 && i + kIdxBlock <= end)
+		{
+			const size_t k = i / kIdxBlock;
+			lo = b->blkMin[k] < lo ? b->blkMin[k] : lo;
+			hi = b->blkMax[k] > hi ? b->blkMax[k] : hi;
+			i += kIdxBlock;
+			continue;
+		}
+		const uint64_t v = hostIndex(b, i);
+		lo = v < lo ? v : lo;
+		hi = v > hi ? v : hi;
+		i++;
+	}
+}
+
+}  // namespace
+
+/* ======================================================================= */
+extern "C" {
+
+/* ---------------------------------------------------------------- context */
+
+/* core/context.c:15-51 */
+void srpNewContext(SRPContext* c)
+{
+	g_ctx = c;
+	c->messageCallback.func = nullptr;
+	c->messageCallback.userParameter = nullptr;
+	c->provokingVertexMode = SRP_PROVOKING_VERTEX_LAST;
+	c->raster.frontFace = SRP_WINDING_CCW;
+	c->raster.cullFace = SRP_FACE_NONE;
+	c->raster.polygonMode = SRP_POLYGON_MODE_FILL;
+	c->raster.pointSize = 1.0f;
+	memset(&c->scissor, 0, sizeof(c->scissor));
+	c->depth.testEnable = false;
+	c->depth.writeEnable = true;
+	c->depth.compareOp = SRP_COMPARE_GREATER;
+	SRPStencilFaceState f;
+	f.func = SRP_COMPARE_ALWAYS;
+	f.ref = 0; f.mask = 0xFF; f.writeMask = 0xFF;
+	f.sfailOp = SRP_STENCIL_KEEP; f.dfailOp = SRP_STENCIL_KEEP; f.passOp = SRP_STENCIL_KEEP;
+	c->stencil.enabled = false;
+	c->stencil.front = f;
+	c->stencil.back = f;
+	c->arena = (SRPArena*) &rt();
+}
+
+/* core/context.c:53-180 */
+void srpSetMessageCallback(SRPMessageCallback cb) { ctx()->messageCallback = cb; }
+void srpProvokingVertexMode(SRPProvokingVertexMode m) { ctx()->provokingVertexMode = m; }
+void srpRasterCullFace(SRPFace f) { ctx()->raster.cullFace = f; }
+void srpRasterFrontFace(SRPWinding w) { ctx()->raster.frontFace = w; }
+void srpRasterPolygonMode(SRPPolygonMode m) { ctx()->raster.polygonMode = m; }
+void srpRasterPointSize(float s) { ctx()->raster.pointSize = s; }
+void srpScissorTest(bool e) { ctx()->scissor.enabled = e; }
+void srpScissorOptions(size_t x, size_t y, size_t w, size_t h)
+{
+	SRPContext* c = ctx();
+	c->scissor.x = x; c->scissor.y = y; c->scissor.width = w; c->scissor.height = h;
+}
+void srpStencilTest(bool) { ctx()->stencil.enabled = true; }
+void srpStencilFunc(SRPCompareOp func, uint8_t ref, uint8_t mask)
+{
+	SRPContext* c = ctx();
+	c->stencil.front.func = func; c->stencil.front.ref = ref; c->stencil.front.mask = mask;
+	c->stencil.back.func = func; c->stencil.back.ref = ref; c->stencil.back.mask = mask;
+}
+static SRPStencilFaceState* faceState(SRPFace f)
+{
+	return (f == SRP_FACE_FRONT) ? &ctx()->stencil.front : &ctx()->stencil.back;
+}
+void srpStencilFuncSeparate(SRPFace face, SRPCompareOp func, uint8_t ref, uint8_t mask)
+{
+	if (face == SRP_FACE_NONE) return;
+	if (face == SRP_FACE_FRONT_AND_BACK) srpStencilFunc(func, ref, mask);
+	SRPStencilFaceState* s = faceState(face);
+	s->func = func; s->ref = ref; s->mask = mask;
+}
+void srpStencilOp(SRPStencilOp sfail, SRPStencilOp dfail, SRPStencilOp pass)
+{
+	SRPContext* c = ctx();
+	c->stencil.front.sfailOp = sfail; c->stencil.front.dfailOp = dfail; c->stencil.front.passOp = pass;
+	c->stencil.back.sfailOp = sfail; c->stencil.back.dfailOp = dfail; c->stencil.back.passOp = pass;
+}
+void srpStencilOpSeparate(SRPFace face, SRPStencilOp sfail, SRPStencilOp dfail, SRPStencilOp pass)
+{
+	if (face == SRP_FACE_NONE) return;
+	if (face == SRP_FACE_FRONT_AND_BACK) srpStencilOp(sfail, dfail, pass);
+	SRPStencilFaceState* s = faceState(face);
+	s->sfailOp = sfail; s->dfailOp = dfail; s->passOp = pass;
+}
+void srpStencilWriteMask(uint8_t m)
+{
+	ctx()->stencil.front.writeMask = m;
+	ctx()->stencil.back.writeMask = m;
+}
+void srpStencilWriteMaskSeparate(SRPFace face, uint8_t m)
+{
+	if (face == SRP_FACE_NONE) return;
+	if (face == SRP_FACE_FRONT_AND_BACK) srpStencilWriteMask(m);
+	faceState(face)->writeMask = m;
+}
+void srpDepthTest(bool e) { ctx()->depth.testEnable = e; }
+void srpDepthWrite(bool e) { ctx()->depth.writeEnable = e; }
+void srpDepthCompareOp(SRPCompareOp op) { ctx()->depth.compareOp = op; }
+
+/* ---------------------------------------------------------------- buffers */
+
+SRPVertexBuffer* srpNewVertexBuffer(void) { return new SRPVertexBuffer(); }
+
+void srpFreeVertexBuffer(SRPVertexBuffer* b)
+{
+	if (!b) return;
+	if (b->dData)
+	{
+		HIP_CHECK(hipStreamSynchronize(rt().stream));
+		HIP_CHECK(hipFree(b->dData));
+	}
+	delete b;
+}
+
+/* core/buffer.c:30-45 (the copy lands in device memory) */
+void srpVertexBufferCopyData(SRPVertexBuffer* b, size_t stride, size_t nBytes, const void* data)
+{
+	Runtime& R = rt();
+	if (nBytes > b->nBytesAllocated)
+	{
+		if (b->dData)
+		{
+			HIP_CHECK(hipStreamSynchronize(R.stream));
+			HIP_CHECK(hipFree(b->dData));
+		}
+		/* slack: shaders may read a little past the last vertex struct */
+		HIP_CHECK(hipMalloc((void**) &b->dData, nBytes + 256));
+		b->nBytesAllocated = nBytes;
+	}
+	b->nBytesPerVertex = stride;
+	b->nVertices = stride ? nBytes / stride : 0;
+	HIP_CHECK(hipMemcpyAsync(b->dData, data, nBytes, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+}
+
+SRPIndexBuffer* srpNewIndexBuffer(void) { return new SRPIndexBuffer(); }
+
+void srpFreeIndexBuffer(SRPIndexBuffer* b)
+{
+	if (!b) return;
+	if (b->dData)
+	{
+		HIP_CHECK(hipStreamSynchronize(rt().stream));
+		HIP_CHECK(hipFree(b->dData));
+	}
+	delete b;
+}
+
+/* utils/type.c:15-40 */
+static size_t sizeofIndexType(SRPType t)
+{
+	switch (t)
+	{
+		case SRP_UINT8: return 1;
+		case SRP_UINT16: return 2;
+		case SRP_UINT32: return 4;
+		case SRP_UINT64: return 8;
+		case SRP_FLOAT: return sizeof(float);
+		case SRP_DOUBLE: return sizeof(double);
+		default:
+			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpSizeofType", "Unknown type (%i)", t);
+			return 0;
+	}
+}
+
+/* core/buffer.c:77-94 */
+void srpIndexBufferCopyData(SRPIndexBuffer* b, SRPType type, size_t nBytes, const void* data)
+{
+	Runtime& R = rt();
+	if (nBytes > b->nBytesAllocated)
+	{
+		if (b->dData)
+		{
+			HIP_CHECK(hipStreamSynchronize(R.stream));
+			HIP_CHECK(hipFree(b->dData));
+		}
+		HIP_CHECK(hipMalloc((void**) &b->dData, nBytes + 16));
+		b->nBytesAllocated = nBytes;
+	}
+	b->type = type;
+	b->elem = sizeofIndexType(type);
+	b->nIndices = b->elem ? nBytes / b->elem : 0;
+	b->host.assign((const uint8_t*) data, (const uint8_t*) data + nBytes);
+	const size_t nb = (b->nIndices + kIdxBlock - 1) / kIdxBlock;
+	b->blkMin.assign(nb, UINT64_MAX);
+	b->blkMax.assign(nb, 0);
+	if (b->elem == 1 || b->elem == 2 || b->elem == 4 || b->elem == 8)
+		for (size_t i = 0; i < b->nIndices; i++)
+		{
+			const uint64_t v = hostIndex(b, i);
+			uint64_t& lo = b->blkMin[i / kIdxBlock];
+			uint64_t& hi = b->blkMax[i / kIdxBlock];
+			lo = v < lo ? v : lo;
+			hi = v > hi ? v : hi;
+		}
+	HIP_CHECK(hipMemcpyAsync(b->dData, data, nBytes, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+}
+
+/* ------------------------------------------------------------ framebuffer */
+
+SRPFramebuffer* srpNewFramebuffer(size_t w, size_t h)
+{
+	Runtime& R = rt();
+	FbImpl* F = new FbImpl();
+	F->pub.width = w;
+	F->pub.height = h;
+	F->pub.size = w * h;
+	const size_t n = F->pub.size;
+	HIP_CHECK(hipHostMalloc((void**) &F->pub.color, n * 4 + 16, hipHostMallocDefault));
+	HIP_CHECK(hipHostMalloc((void**) &F->pub.depth, n * 4 + 16, hipHostMallocDefault));
+	HIP_CHECK(hipHostMalloc((void**) &F->pub.stencil, n + 16, hipHostMallocDefault));
+	HIP_CHECK(hipMalloc((void**) &F->dColor, n * 4 + 16));
+	HIP_CHECK(hipMalloc((void**) &F->dDepth, n * 4 + 16));
+	HIP_CHECK(hipMalloc((void**) &F->dStencil, n + 16));
+	/* Stencil starts zeroed (the reference's tests rely on fresh zero pages). */
+	memset(F->pub.stencil, 0, n);
+	memset(F->pub.color, 0, n * 4);
+	memset(F->pub.depth, 0, n * 4);
+	fillU8(F->dStencil, 0, n, R.stream);
+	fillU32(F->dColor, 0, n, R.stream);
+	fillU32((uint32_t*) F->dDepth, 0, n, R.stream);
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	return &F->pub;
+}
+
+void srpFreeFramebuffer(SRPFramebuffer* fb)
+{
+	if (!fb) return;
+	FbImpl* F = fbImpl(fb);
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+	HIP_CHECK(hipHostFree(F->pub.color));
+	HIP_CHECK(hipHostFree(F->pub.depth));
+	HIP_CHECK(hipHostFree(F->pub.stencil));
+	HIP_CHECK(hipFree(F->dColor));
+	HIP_CHECK(hipFree(F->dDepth));
+	HIP_CHECK(hipFree(F->dStencil));
+	delete F;
+}
+
+/* core/framebuffer.c:57-62: colour 0, depth -1, stencil untouched */
+void srpFramebufferClear(const SRPFramebuffer* fb)
+{
+	FbImpl* F = fbImpl(fb);
+	if (F->deferred)
+	{
+		Runtime& R = rt();
+		fillU32(F->dColor, 0u, fb->size, R.stream);
+		const float m1 = -1.0f;
+		uint32_t bits;
+		memcpy(&bits, &m1, 4);
+		fillU32((uint32_t*) F->dDepth, bits, fb->size, R.stream);
+		return;
+	}
+	memset(fb->color, 0, fb->size * 4);
+	for (size_t i = 0; i < fb->size; i++) fb->depth[i] = -1.0f;
+}
+
+void srpFramebufferSetDeferred(const SRPFramebuffer* fb, bool deferred)
+{
+	FbImpl* F = fbImpl(fb);
+	if (deferred && !F->deferred)
+		fbUpload(F);
+	if (!deferred && F->deferred)
+		fbDownload(F);
+	F->deferred = deferred;
+}
+
+void srpFramebufferDownload(const SRPFramebuffer* fb) { fbDownload(fbImpl(fb)); }
+
+void srpFramebufferUpload(const SRPFramebuffer* fb)
+{
+	fbUpload(fbImpl(fb));
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void srpFramebufferFillDepth(const SRPFramebuffer* fb, float v)
+{
+	FbImpl* F = fbImpl(fb);
+	if (F->deferred)
+	{
+		uint32_t bits;
+		memcpy(&bits, &v, 4);
+		fillU32((uint32_t*) F->dDepth, bits, fb->size, rt().stream);
+		return;
+	}
+	for (size_t i = 0; i < fb->size; i++) fb->depth[i] = v;
+}
+
+void srpFramebufferFillStencil(const SRPFramebuffer* fb, uint8_t v)
+{
+	FbImpl* F = fbImpl(fb);
+	if (F->deferred)
+	{
+		fillU8(F->dStencil, v, fb->size, rt().stream);
+		return;
+	}
+	memset(fb->stencil, v, fb->size);
+}
+
+/* --------------------------------------------------------------- textures */
+
+/* core/texture.c:27-48 */
+SRPTexture* srpNewTexture(const char* path, SRPTextureWrappingMode wx, SRPTextureWrappingMode wy)
+{
+	int w = 0, h = 0;
+	const char* err = "unknown error";
+	uint8_t* rgb = srpPngLoadRGB(path, &w, &h, &err);
+	if (!rgb)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewTexture", "Failed to load image `%s`: %s", path, err);
+		return nullptr;
+	}
+	SRPTexture* t = makeTexture(rgb, w, h, wx, wy);
+	free(rgb);
+	return t;
+}
+
+SRPTexture* srpNewTextureFromMemory(const uint8_t* rgb, int w, int h, SRPTextureWrappingMode wx, SRPTextureWrappingMode wy)
+{
+	if (!rgb || w <= 0 || h <= 0)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewTextureFromMemory", "Invalid texture data");
+		return nullptr;
+	}
+	return makeTexture((uint8_t*) rgb, w, h, wx, wy);
+}
+
+void srpFreeTexture(SRPTexture* t)
+{
+	TexImpl* T = texImpl(t);
+	if (!T) return;
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+	HIP_CHECK(hipFree(T->dTexels));
+	HIP_CHECK(hipFree(T->dDesc));
+	textures().erase((const void*) t);
+	delete T;
+}
+
+/* core/texture.c:56-81 */
+void srpTextureGetFilteredColorHost(const SRPTexture* t, float u, float v, float out[4])
+{
+	const TexImpl* T = texImpl(t);
+	if (!T)
+	{
+		out[0] = out[1] = out[2] = 0.0f;
+		out[3] = 1.0f;
+		return;
+	}
+	if (u < 0 || u > 1)
+		u = (T->desc.wrapX == TW_REPEAT) ? (float) ((double) u - floor((double) u)) : (float) fmax(0.0, fmin(1.0, (double) u));
+	if (v < 0 || v > 1)
+		v = (T->desc.wrapY == TW_REPEAT) ? (float) ((double) v - floor((double) v)) : (float) fmax(0.0, fmin(1.0, (double) v));
+	const float x = (float) T->desc.wM1 * u;
+	const float y = (float) T->desc.hM1 * (1 - v);
+	const double xd = (double) x + 0.5, yd = (double) y + 0.5;
+	int64_t xi = (xd >= 0.0 && xd < 9.0e18) ? (int64_t) xd : 0;
+	int64_t yi = (yd >= 0.0 && yd < 9.0e18) ? (int64_t) yd : 0;
+	if (xi > T->desc.wM1) xi = T->desc.wM1;
+	if (yi > T->desc.hM1) yi = T->desc.hM1;
+	const uint8_t* p = T->rgb.data() + (size_t) (yi * T->w + xi) * 3;
+	const float inv255 = (float) (1. / 255.);
+	out[0] = p[0] * inv255;
+	out[1] = p[1] * inv255;
+	out[2] = p[2] * inv255;
+	out[3] = 1.0f;
+}
+
+/* core/texture.c:83-137 */
+int srpTextureGet(SRPTexture* t, SRPTextureParameter p)
+{
+	TexImpl* T = texImpl(t);
+	if (T && p == SRP_TEXTURE_WRAPPING_MODE_X) return T->desc.wrapX;
+	if (T && p == SRP_TEXTURE_WRAPPING_MODE_Y) return T->desc.wrapY;
+	message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpTextureGet", "Unknown texture parameter (%i)", p);
+	return -1;
+}
+
+void srpTextureSet(SRPTexture* t, SRPTextureParameter p, int v)
+{
+	TexImpl* T = texImpl(t);
+	if (!T || (p != SRP_TEXTURE_WRAPPING_MODE_X && p != SRP_TEXTURE_WRAPPING_MODE_Y))
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpTextureSet", "Unknown texture parameter (%i)", p);
+		return;
+	}
+	if (p == SRP_TEXTURE_WRAPPING_MODE_X) T->desc.wrapX = v;
+	else T->desc.wrapY = v;
+	texPush(T);
+	if (v != TW_REPEAT && v != TW_CLAMP_TO_EDGE)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpTextureSet",
+		        "Unknown texture wrapping mode (%i). Falling back to TW_REPEAT", v);
+}
+
+/* ----------------------------------------------------- registration, misc */
+
+void srpHipRegisterVertexShader(const void* fn, const void* entry)
+{
+	std::lock_guard<std::mutex> g(g_regLock);
+	const SrpVsEntry* e = (const SrpVsEntry*) entry;
+	if (e->abiVersion != SRP_ABI_VERSION)
+	{
+		fprintf(stderr, "libsrp_amd: vertex shader module built for ABI %u, library is %u\n", e->abiVersion, SRP_ABI_VERSION);
+		return;
+	}
+	vsRegistry()[fn] = e;
+}
+
+void srpHipRegisterFragmentShader(const void* fn, const void* entry)
+{
+	std::lock_guard<std::mutex> g(g_regLock);
+	const SrpFsEntry* e = (const SrpFsEntry*) entry;
+	if (e->abiVersion != SRP_ABI_VERSION)
+	{
+		fprintf(stderr, "libsrp_amd: fragment shader module built for ABI %u, library is %u\n", e->abiVersion, SRP_ABI_VERSION);
+		return;
+	}
+	fsRegistry()[fn] = e;
+}
+
+void srpHipShaderCalledOnHost(const char* name)
+{
+	message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, name,
+	        "device shader `%s` was called from host code; it only runs inside srpDraw*", name);
+}
+
+void srpHipGetStats(uint64_t out[4])
+{
+	Runtime& R = rt();
+	unsigned long long d[4];
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	HIP_CHECK(hipMemcpy(d, R.dStats, sizeof(d), hipMemcpyDeviceToHost));
+	out[0] = d[0];
+	out[1] = d[1];
+	out[2] = R.hostPrims;
+	out[3] = R.hostDraws;
+}
+
+void srpHipResetStats(void)
+{
+	Runtime& R = rt();
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	HIP_CHECK(hipMemset(R.dStats, 0, 4 * sizeof(unsigned long long)));
+	R.hostPrims = 0;
+	R.hostDraws = 0;
+}
+
+void* srpHipGetStream(void) { return (void*) rt().stream; }
+void srpHipSynchronize(void) { HIP_CHECK(hipStreamSynchronize(rt().stream)); }
+
+void srpHipProfileEnable(bool enable)
+{
+	Runtime& R = rt();
+	R.profile = enable;
+	for (int i = 0; i < 4; i++) { R.profMs[i] = 0; R.profLaunches[i] = 0; }
+}
+
+void srpHipProfileRead(double ms[4], uint64_t launches[4])
+{
+	Runtime& R = rt();
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	for (int i = 0; i < 4; i++) { ms[i] = R.profMs[i]; launches[i] = R.profLaunches[i]; }
+}
+
+void srpHipSetTileSize(int t)
+{
+	if (t == 8 || t == 16) rt().tile = t;
+}
+int srpHipGetTileSize(void) { return rt().tile; }
+
+}  // extern "C"
+
+/* =================================================================== draw */
+
+namespace {
+
+size_t elemSizeOf(uint32_t type)
+{
+	switch (type)
+	{
+		case SRP_FLOAT: case SRP_INT32: case SRP_UINT32: return 4;
+		case SRP_DOUBLE: case SRP_INT64: case SRP_UINT64: return 8;
+		case SRP_INT8: case SRP_UINT8: return 1;
+		case SRP_INT16: case SRP_UINT16: return 2;
+		default: return 0;
+	}
+}
+
+int bitsFor(uint64_t n)
+{
+	int b = 0;
+	while ((1ull << b) < n) b++;
+	return b;
+}
+
+void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFramebuffer* fb,
+                const SRPShaderProgram* sp, SRPPrimitive prim, size_t start, size_t count)
+{
+	SRPContext* C = ctx();
+	/* pipeline/draw.c:63-82, :167-189 */
+	if (count == 0)
+		return;
+	const size_t end = start + count - 1;
+	const size_t size = ib ? ib->nIndices : vb->nVertices;
+	if (end >= size)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "checkOOB",
+		        ib ? "Attempt to OOB access index buffer (read) at indices %zu-%zu (size: %zu)\n"
+		           : "Attempt to OOB access vertex buffer (read) at indices %zu-%zu (size: %zu)\n",
+		        start, end, size);
+		return;
+	}
+	int kind;
+	if (prim == SRP_PRIM_TRIANGLES || prim == SRP_PRIM_TRIANGLE_STRIP || prim == SRP_PRIM_TRIANGLE_FAN) kind = 0;
+	else if (prim == SRP_PRIM_LINES || prim == SRP_PRIM_LINE_STRIP || prim == SRP_PRIM_LINE_LOOP) kind = 1;
+	else if (prim == SRP_PRIM_POINTS) kind = 2;
+	else
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "Unknown primitive type: %i", prim);
+		return;
+	}
+
+	size_t nIn = 0;
+	if (kind == 0)
+	{
+		/* draw.c:89-90 */
+		if (C->raster.cullFace == SRP_FACE_FRONT_AND_BACK)
+			return;
+		if (prim == SRP_PRIM_TRIANGLES && count % 3 != 0)
+			message(SRP_MESSAGE_WARNING, SRP_MESSAGE_SEVERITY_LOW, "warnOnExcessVertexCount",
+			        "Vertex count not divisible by 3 when drawing SRP_PRIM_TRIANGLES. The last %i vertex/vertices will be ignored\n",
+			        (int) (count % 3));
+		nIn = (prim == SRP_PRIM_TRIANGLES) ? count / 3 : (count >= 3 ? count - 2 : 0);
+	}
+	else if (kind == 1)
+	{
+		if (prim == SRP_PRIM_LINES && count % 2 != 0)
+			message(SRP_MESSAGE_WARNING, SRP_MESSAGE_SEVERITY_LOW, "warnOnExcessVertexCount",
+			        "Odd vertex count when drawing SRP_PRIM_LINES. The last vertex will be ignored\n");
+		if (prim == SRP_PRIM_LINES) nIn = count / 2;
+		else if (prim == SRP_PRIM_LINE_STRIP) nIn = (count != 1) ? count - 1 : 0;
+		else nIn = (count != 1) ? count : 0;
+	}
+	else
+	{
+		if (C->raster.pointSize <= 0.)
+			return;
+		nIn = count;
+	}
+	if (nIn == 0)
+		return;
+
+	const SrpVsEntry* vse = nullptr;
+	const SrpFsEntry* fse = nullptr;
+	{
+		std::lock_guard<std::mutex> g(g_regLock);
+		auto iv = vsRegistry().find((const void*) sp->vs->shader);
+		auto ifs = fsRegistry().find((const void*) sp->fs->shader);
+		if (iv != vsRegistry().end()) vse = iv->second;
+		if (ifs != fsRegistry().end()) fse = ifs->second;
+	}
+	if (!vse || !fse)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer",
+		        "%s shader is not a registered device shader (see SRP_HIP_*_SHADER in srp/srp_device.h)",
+		        vse ? "fragment" : "vertex");
+		return;
+	}
+
+	Runtime& R = rt();
+	FbImpl* F = fbImpl(fb);
+	if (fb->width > 65535 || fb->height > 65535 || fb->size >= 0xFFFFFFFFull)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "Framebuffer too large for the GPU path");
+		return;
+	}
+
+	/* Varying layout (interpolation.c:93-163).  The staging block may still
+	 * be read by the previous draw's last parameter copy. */
+	HIP_CHECK(hipEventSynchronize(R.paramsCopied));
+	SrpDrawParams& H = *R.hParams;
+	memset(&H, 0, sizeof(H));
+	const SRPVertexShader* V = sp->vs;
+	if (V->nVaryings > SRP_MAX_VARYINGS)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "At most %d varyings are supported", SRP_MAX_VARYINGS);
+		return;
+	}
+	uint32_t off = 0;
+	bool words = (V->varyingsSize % 4) == 0;
+	uint32_t nWords = 0;
+	for (size_t a = 0; a < V->nVaryings; a++)
+	{
+		const SRPVaryingInfo& vi = V->varyingsInfo[a];
+		const size_t es = elemSizeOf(vi.type);
+		if (es == 0)
+			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "interpolateAttributes", "Unexpected type (%i)", vi.type);
+		H.var[a].nItems = (uint32_t) vi.nItems;
+		H.var[a].type = vi.type;
+		H.var[a].mode = vi.interpolationMode;
+		H.var[a].offset = off;
+		const bool fourByte = vi.type == SRP_FLOAT || vi.type == SRP_INT32 || vi.type == SRP_UINT32;
+		if (!fourByte || (off % 4) != 0)
+			words = false;
+		for (size_t e = 0; e < vi.nItems && words; e++)
+		{
+			if (nWords >= SRP_MAX_WORDS) { words = false; break; }
+			uint8_t k = SRP_WORD_PROVOKING;
+			if (vi.type == SRP_FLOAT && vi.interpolationMode == SRP_INTERPOLATION_MODE_PERSPECTIVE) k = SRP_WORD_PERSPECTIVE;
+			else if (vi.type == SRP_FLOAT && vi.interpolationMode == SRP_INTERPOLATION_MODE_AFFINE) k = SRP_WORD_AFFINE;
+			H.wordKind[nWords++] = k;
+		}
+		off += (uint32_t) (es * vi.nItems);
+	}
+	if (off > SRP_MAX_VARYING_BYTES)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "Varyings larger than %d bytes are not supported",
+		        SRP_MAX_VARYING_BYTES);
+		return;
+	}
+	H.nVaryings = (uint32_t) V->nVaryings;
+	H.describedSize = off;
+	H.varyingsSize = (uint32_t) V->varyingsSize;
+	H.wordPath = words ? 1u : 0u;
+	H.nWords = words ? nWords : 0u;
+
+	/* Post-VS cache range (vertex_processing.c:23-36). */
+	uint64_t lo, hi;
+	if (ib)
+	{
+		indexRange(ib, start, count, lo, hi);
+		if (hi >= vb->nVertices)
+		{
+			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer",
+			        "Index %llu exceeds the vertex buffer (%zu vertices)", (unsigned long long) hi, vb->nVertices);
+			return;
+		}
+	}
+	else
+	{
+		lo = start;
+		hi = start + count - 1;
+	}
+	if (hi - lo + 1 > 0x7FFFFFFFull)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "Vertex index range too large");
+		return;
+	}
+	const uint32_t cacheSize = (uint32_t) (hi - lo + 1);
+
+	/* State snapshot. */
+	SrpState& S = H.st;
+	S.provokingLast = C->provokingVertexMode == SRP_PROVOKING_VERTEX_LAST;
+	S.cullFace = C->raster.cullFace;
+	S.frontFace = C->raster.frontFace;
+	S.polygonMode = C->raster.polygonMode;
+	if (kind == 0 && S.polygonMode > SRP_POLYGON_MODE_POINT)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "assembleTrianglesGeneric",
+		        "Unexpected srpContext.raster.polygonMode (%i)", C->raster.polygonMode);
+		return;
+	}
+	S.pointSize = C->raster.pointSize;
+	S.scissorEnabled = C->scissor.enabled;
+	S.sx0 = C->scissor.x;
+	S.sx1 = C->scissor.x + C->scissor.width;
+	S.sy0 = C->scissor.y;
+	S.sy1 = C->scissor.y + C->scissor.height;
+	S.stencilEnabled = C->stencil.enabled;
+	S.stencil[0] = C->stencil.front;
+	S.stencil[1] = C->stencil.back;
+	S.depthTest = C->depth.testEnable;
+	S.depthWrite = C->depth.writeEnable;
+	S.depthOp = C->depth.compareOp;
+	S.lateZ = sp->fs->mayOverwriteDepth;
+
+	H.width = (uint32_t) fb->width;
+	H.height = (uint32_t) fb->height;
+	H.color = F->dColor;
+	H.depth = F->dDepth;
+	H.stencil = F->dStencil;
+
+	const size_t usize = vse->uniformSize > fse->uniformSize ? vse->uniformSize : fse->uniformSize;
+	if (sp->uniform && usize)
+	{
+		if (usize > R.uniformCap)
+		{
+			if (R.hUniform) HIP_CHECK(hipHostFree(R.hUniform));
+			R.uniformCap = usize * 2;
+			HIP_CHECK(hipHostMalloc((void**) &R.hUniform, R.uniformCap, hipHostMallocDefault));
+		}
+		memcpy(R.hUniform, sp->uniform, usize);
+		uint8_t* dU = R.uniform.get<uint8_t>(usize + 16);
+		HIP_CHECK(hipMemcpyAsync(dU, R.hUniform, usize, hipMemcpyHostToDevice, R.stream));
+		H.uniform = dU;
+	}
+	else
+		H.uniform = nullptr;
+
+	H.vertices = vb->dData;
+	H.vertexStride = vb->nBytesPerVertex;
+	H.cacheBase = lo;
+	H.cacheSize = cacheSize;
+	H.clip = R.clip.get<float>((size_t) cacheSize * 16);
+	H.varyings = R.varyings.get<uint8_t>((size_t) cacheSize * V->varyingsSize + 512);
+	H.genVaryings = H.varyings;
+	H.tileShift = (R.tile == 16) ? 4u : 3u;
+	H.tilesX = (H.width + R.tile - 1) / R.tile;
+	H.tilesY = (H.height + R.tile - 1) / R.tile;
+	H.stats = R.dStats;
+
+	if (!F->deferred)
+		fbUpload(F);
+	if (R.profile) HIP_CHECK(hipEventRecord(R.ev[2], R.stream));
+	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
+
+	/* Vertex stage. */
+	SrpVsLaunch vl;
+	vl.params = R.dParams;
+	vl.count = cacheSize;
+	vse->launch(&vl, (void*) R.stream);
+
+	/* Assembly COUNT pass + scans. */
+	uint32_t* counts = R.counts.get<uint32_t>((nIn + 1) * 3 * sizeof(uint32_t));
+	uint32_t* totals = R.totals.get<uint32_t>(8 * sizeof(uint32_t));
+	fillU32(totals, 0u, 8, R.stream);
+	AsmArgs A;
+	memset(&A, 0, sizeof(A));
+	A.P = R.dParams;
+	A.prim = prim;
+	A.nIn = (uint32_t) nIn;
+	A.start = start;
+	A.count = count;
+	A.indices = ib ? ib->dData : nullptr;
+	A.indexElem = ib ? (uint32_t) ib->elem : 0u;
+	A.recOff = counts;
+	A.genOff = counts + (nIn + 1);
+	A.itemOff = counts + 2 * (nIn + 1);
+	A.error = totals + 3;
+	launchAssemble(A, kind, false, R.stream);
+	scanExclusive(A.recOff, nIn, totals + 0, R.scan, R.stream);
+	scanExclusive(A.genOff, nIn, totals + 1, R.scan, R.stream);
+	scanExclusive(A.itemOff, nIn, totals + 2, R.scan, R.stream);
+	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	const uint32_t nRec = R.hTotals[0], nGen = R.hTotals[1], nItems = R.hTotals[2];
+	if (R.hTotals[3])
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
+		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
+	R.hostDraws++;
+	R.hostPrims += nRec;
+	if (nRec == 0)
+		return;   /* nothing drawn: the device copy is unchanged */
+
+	/* EMIT pass. */
+	const int backend = (kind == 0) ? (int) S.polygonMode : (kind == 1 ? 1 : 2);  /* 0 tiles, 1 lines, 2 points */
+	H.genVaryings = R.genVaryings.get<uint8_t>((size_t) nGen * V->varyingsSize + 512);
+	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
+	A.genVaryings = H.genVaryings;
+	const uint32_t nTiles = H.tilesX * H.tilesY;
+	uint32_t* tileCount = nullptr;
+	if (backend == 0)
+	{
+		A.tris = R.tris.get<SrpTri>((size_t) nRec * sizeof(SrpTri));
+		A.pairKeys = R.pairKeys.get<uint32_t>((size_t) nItems * 4 + 16);
+		A.pairVals = R.pairVals.get<uint32_t>((size_t) nItems * 4 + 16);
+		tileCount = R.tileCount.get<uint32_t>(((size_t) nTiles + 2) * 4);
+		fillU32(tileCount, 0u, (size_t) nTiles + 1, R.stream);
+		A.tileCount = tileCount;
+	}
+	else
+	{
+		if (backend == 1) A.lines = R.lines.get<SrpLine>((size_t) nRec * sizeof(SrpLine));
+		else A.points = R.points.get<SrpPoint>((size_t) nRec * sizeof(SrpPoint));
+		A.frags = R.frags.get<SrpFrag>((size_t) nItems * sizeof(SrpFrag) + 16);
+		A.fragKeys = R.fragKeys.get<uint32_t>((size_t) nItems * 4 + 16);
+	}
+	launchAssemble(A, kind, true, R.stream);
+
+	if (backend == 0)
+	{
+		uint32_t *keys, *vals;
+		radixSort(A.pairKeys, A.pairVals, nItems, bitsFor(nTiles), R.sort, &keys, &vals, R.stream);
+		scanExclusive(tileCount, (size_t) nTiles + 1, totals + 4, R.scan, R.stream);
+		SrpRasterLaunch rl;
+		rl.params = R.dParams;
+		rl.tris = A.tris;
+		rl.tileList = vals;
+		rl.tileStart = tileCount;
+		rl.nTiles = nTiles;
+		rl.lateZ = S.lateZ;
+		rl.words = H.wordPath;
+		rl.tile = (uint32_t) R.tile;
+		const uint32_t perBlock = (R.tile == 8) ? 4u : 1u;
+		rl.nBlocks = (nTiles + perBlock - 1) / perBlock;
+		if (R.profile) HIP_CHECK(hipEventRecord(R.ev[0], R.stream));
+		fse->raster(&rl, (void*) R.stream);
+		if (R.profile) HIP_CHECK(hipEventRecord(R.ev[1], R.stream));
+	}
+	else
+	{
+		uint32_t* order = R.order.get<uint32_t>((size_t) nItems * 4 + 16);
+		iota(order, nItems, R.stream);
+		uint32_t *keys, *vals;
+		radixSort(A.fragKeys, order, nItems, bitsFor((uint64_t) H.width * H.height + 1), R.sort, &keys, &vals, R.stream);
+		SrpFragLaunch fl;
+		fl.params = R.dParams;
+		fl.frags = A.frags;
+		fl.keys = keys;
+		fl.order = vals;
+		fl.lines = A.lines;
+		fl.points = A.points;
+		fl.count = nItems;
+		fl.kind = (backend == 1) ? 0u : 1u;
+		fl.lateZ = S.lateZ;
+		fl.words = H.wordPath;
+		fse->frags(&fl, (void*) R.stream);
+	}
+	if (R.profile)
+	{
+		HIP_CHECK(hipEventRecord(R.ev[3], R.stream));
+		HIP_CHECK(hipEventSynchronize(R.ev[3]));
+		float ms = 0.0f;
+		if (backend == 0)
+		{
+			HIP_CHECK(hipEventElapsedTime(&ms, R.ev[0], R.ev[1]));
+			R.profMs[0] += ms;
+			R.profLaunches[0]++;
+		}
+		HIP_CHECK(hipEventElapsedTime(&ms, R.ev[2], R.ev[3]));
+		R.profMs[1] += ms;
+		R.profLaunches[1]++;
+	}
+	if (!F->deferred)
+		fbDownload(F);
+}
+
+}  // namespace
+
+extern "C" {
+
+/* core/buffer.c:53-59 */
+void srpDrawVertexBuffer(const SRPVertexBuffer* vb, const SRPFramebuffer* fb, const SRPShaderProgram* sp,
+                         SRPPrimitive prim, size_t start, size_t count)
+{
+	drawBuffer(nullptr, vb, fb, sp, prim, start, count);
+}
+
+/* core/buffer.c:130-136 */
+void srpDrawIndexBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFramebuffer* fb,
+                        const SRPShaderProgram* sp, SRPPrimitive prim, size_t start, size_t count)
+{
+	drawBuffer(ib, vb, fb, sp, prim, start, count);
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+/* Data-parallel building blocks of the pipeline: exclusive scan, a stable
+ * LSD radix sort used for ordered binning (tile lists, per-pixel fragment
+ * lists), fills. */
+#include "kernels.h"
+
+namespace srp {
+namespace lib {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_local(uint32_t* data, size_t n, uint32_t* sums)
+{
+	__shared__ uint32_t sh[SCAN_THREADS];
+	const uint32_t t = threadIdx.x;
+	const size_t base = (size_t) blockIdx.x * SCAN_CHUNK + (size_t) t * SCAN_ITEMS;
+	uint32_t v[SCAN_ITEMS];
+	uint32_t sum = 0;
+#pragma unroll
+	for (int j = 0; j < SCAN_ITEMS; j++)
+	{
+		const size_t i = base + j;
+		v[j] = (i < n) ? data[i] : 0u;
+		sum += v[j];
+	}
+	sh[t] = sum;
+	__syncthreads();
+	for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1)
+	{
+		const uint32_t x = (t >= off) ? sh[t - off] : 0u;
+		__syncthreads();
+		sh[t] += x;
+		__syncthreads();
+	}
+	uint32_t run = sh[t] - sum;
+#pragma unroll
+	for (int j = 0; j < SCAN_ITEMS; j++)
+	{
+		const size_t i = base + j;
+		if (i < n) data[i] = run;
+		run += v[j];
+	}
+	if (t == SCAN_THREADS - 1)
+		sums[blockIdx.x] = sh[t];
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* data, size_t n, const uint32_t* sums)
+{
+	const uint32_t add = sums[blockIdx.x];
+	const size_t base = (size_t) blockIdx.x * SCAN_CHUNK;
+	for (int j = 0; j < SCAN_ITEMS; j++)
+	{
+		const size_t i = base + (size_t) j * SCAN_THREADS + threadIdx.x;
+		if (i < n) data[i] += add;
+	}
+}
+
+__global__ void k_copy1(const uint32_t* src, uint32_t* dst) { *dst = *src; }
+
+static void scanLevel(uint32_t* data, size_t n, uint32_t* total, uint32_t* scratch, hipStream_t s)
+{
+	const size_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+	uint32_t* sums = scratch;
+	hipLaunchKernelGGL(k_scan_local, dim3((uint32_t) nb), dim3(SCAN_THREADS), 0, s, data, n, sums);
+	if (nb == 1)
+	{
+		hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, s, sums, total);
+		return;
+	}
+	scanLevel(sums, nb, total, scratch + nb, s);
+	hipLaunchKernelGGL(k_scan_add, dim3((uint32_t) nb), dim3(SCAN_THREADS), 0, s, data, n, sums);
+}
+
+void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scratch, hipStream_t s)
+{
+	if (n == 0)
+	{
+		fillU32(total, 0u, 1, s);
+		return;
+	}
+	size_t need = 0;
+	for (size_t m = n; m > 1; m = (m + SCAN_CHUNK - 1) / SCAN_CHUNK)
+		need += (m + SCAN_CHUNK - 1) / SCAN_CHUNK;
+	need += 2;
+	if (need > scratch.cap)
+	{
+		if (scratch.buf) hipFree(scratch.buf);
+		scratch.cap = need * 2;
+		hipMalloc(&scratch.buf, scratch.cap * sizeof(uint32_t));
+	}
+	scanLevel(data, n, total, scratch.buf, s);
+}
+
+/* ------------------------------------------------------------ radix sort */
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_CHUNK = 4096;   /* items per block */
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys, size_t n, uint32_t shift,
+                                                           uint32_t* hist, uint32_t nBlocks)
+{
+	__shared__ uint32_t h[256];
+	const uint32_t t = threadIdx.x;
+	h[t] = 0;
+	__syncthreads();
+	const size_t begin = (size_t) blockIdx.x * RS_CHUNK;
+	const size_t end = (begin + RS_CHUNK < n) ? begin + RS_CHUNK : n;
+	for (size_t i = begin + t; i < end; i += RS_THREADS)
+		atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+	__syncthreads();
+	hist[(size_t) t * nBlocks + blockIdx.x] = h[t];
+}
+
+/* Stable scatter: items keep their input order within a digit.  Each tile
+ * of 256 items ranks its items per digit with wave ballots, then by wave. */
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
+                                                              uint32_t* keysOut, uint32_t* valsOut, size_t n,
+                                                              uint32_t shift, const uint32_t* hist, uint32_t nBlocks)
+{
+	__shared__ uint32_t running[256];
+	__shared__ uint32_t waveCnt[4][256];
+	__shared__ uint32_t tileTotal[256];
+	const uint32_t t = threadIdx.x;
+	const uint32_t lane = t & 63u, wave = t >> 6;
+	running[t] = hist[(size_t) t * nBlocks + blockIdx.x];
+	const size_t begin = (size_t) blockIdx.x * RS_CHUNK;
+	const size_t end = (begin + RS_CHUNK < n) ? begin + RS_CHUNK : n;
+	const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
+	for (size_t base = begin; base < end; base += RS_THREADS)
+	{
+		for (int w = 0; w < 4; w++) waveCnt[w][t] = 0;
+		__syncthreads();
+		const size_t i = base + t;
+		const bool valid = i < end;
+		uint32_t key = 0, val = 0, d = 0;
+		if (valid)
+		{
+			key = keysIn[i];
+			val = valsIn[i];
+			d = (key >> shift) & 255u;
+		}
+		uint64_t peers = __ballot(valid);
+#pragma unroll
+		for (int b = 0; b < 8; b++)
+		{
+			const bool bit = (d >> b) & 1u;
+			const uint64_t m = __ballot(bit);
+			peers &= bit ? m : ~m;
+		}
+		const uint32_t rank = (uint32_t) __popcll(peers & ltMask);
+		if (valid && rank == 0)
+			waveCnt[wave][d] = (uint32_t) __popcll(peers);
+		__syncthreads();
+		{
+			uint32_t sum = 0;
+			for (int w = 0; w < 4; w++)
+			{
+				const uint32_t c = waveCnt[w][t];
+				waveCnt[w][t] = sum;
+				sum += c;
+			}
+			tileTotal[t] = sum;
+		}
+		__syncthreads();
+		if (valid)
+		{
+			const uint32_t dst = running[d] + waveCnt[wave][d] + rank;
+			keysOut[dst] = key;
+			valsOut[dst] = val;
+		}
+		__syncthreads();
+		running[t] += tileTotal[t];
+	}
+}
+
+void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& sc,
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s)
+{
+	*outKeys = keys;
+	*outVals = vals;
+	if (n <= 1 || bits <= 0)
+		return;
+	const uint32_t nBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
+	if (n > sc.cap)
+	{
+		if (sc.keys) hipFree(sc.keys);
+		if (sc.vals) hipFree(sc.vals);
+		sc.cap = n + n / 2;
+		hipMalloc(&sc.keys, sc.cap * sizeof(uint32_t));
+		hipMalloc(&sc.vals, sc.cap * sizeof(uint32_t));
+	}
+	const size_t histN = (size_t) 256 * nBlocks;
+	if (histN + 1 > sc.histCap)
+	{
+		if (sc.hist) hipFree(sc.hist);
+		sc.histCap = (histN + 1) * 2;
+		hipMalloc(&sc.hist, sc.histCap * sizeof(uint32_t));
+	}
+	uint32_t* kIn = keys;
+	uint32_t* vIn = vals;
+	uint32_t* kOut = sc.keys;
+	uint32_t* vOut = sc.vals;
+	for (int shift = 0; shift < bits; shift += 8)
+	{
+		hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, (uint32_t) shift, sc.hist, nBlocks);
+		scanExclusive(sc.hist, histN, sc.hist + histN, sc.scan, s);
+		hipLaunchKernelGGL(k_radix_scatter, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
+		                   (uint32_t) shift, sc.hist, nBlocks);
+		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
+		uint32_t* tv = vIn; vIn = vOut; vOut = tv;
+	}
+	*outKeys = kIn;
+	*outVals = vIn;
+}
+
+/* ------------------------------------------------------------------ fills */
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for (size_t k = i; k < n; k += stride) p[k] = v;
+}
+
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for (size_t k = i; k < n; k += stride) p[k] = v;
+}
+
+__global__ void k_iota(uint32_t* p, size_t n)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for (size_t k = i; k < n; k += stride) p[k] = (uint32_t) k;
+}
+
+static dim3 fillGrid(size_t n)
+{
+	size_t b = (n + 255) / 256;
+	if (b > 4096) b = 4096;
+	if (b == 0) b = 1;
+	return dim3((uint32_t) b);
+}
+
+void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
+{
+	if (n) hipLaunchKernelGGL(k_fill_u32, fillGrid(n), dim3(256), 0, s, p, v, n);
+}
+
+void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s)
+{
+	if (n) hipLaunchKernelGGL(k_fill_u8, fillGrid(n), dim3(256), 0, s, p, v, n);
+}
+
+void iota(uint32_t* p, size_t n, hipStream_t s)
+{
+	if (n) hipLaunchKernelGGL(k_iota, fillGrid(n), dim3(256), 0, s, p, n);
+}
+
+}  // namespace lib
+}  // namespace srp

@@ -20,8 +20,8 @@
 SRPContext srpContext;
 
 /* Shader symbols (host_shaders.c or device_shaders.hip). */
-#define HOST_VS(name, fn) void name(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
-#define HOST_FS(name, fn) void name(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
+#define HOST_VS(name, fn, usize) void name(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
+#define HOST_FS(name, fn, usize) void name(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
 #include "shader_list.h"
 #undef HOST_VS
 #undef HOST_FS

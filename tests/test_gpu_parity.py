@@ -1,0 +1,98 @@
+"""GPU draw pipeline vs the CPU oracle: colour, depth and stencil must be
+bit-identical, and the covered/shaded fragment and primitive counts equal.
+
+The oracle is itself pinned to the compiled reference renderer and to its
+golden images (tests/test_oracle.py); the GPU results are additionally
+checked against the golden PNGs with the reference harness' tolerance of 1
+(tests/compare.py:8 of kitrofimov/srp)."""
+import os
+
+import numpy as np
+import pytest
+
+from scene_runner import REFERENCE_SCENES, REPO, write_noise_png
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(REPO, "tests", "golden", "references")
+
+
+def _stats(mod, which):
+    fn = {"oracle": "srpOracleGetStats", "gpu": "srpHipGetStats"}[which]
+    return mod.stats(fn)
+
+
+def _reset(mod, which):
+    fn = {"oracle": "srpOracleResetStats", "gpu": "srpHipResetStats"}[which]
+    mod.reset_stats(fn)
+
+
+def run_both(oracle, gpu, name, **kw):
+    print(f"[parity] {name} {kw}", flush=True)
+    _reset(oracle, "oracle")
+    _reset(gpu, "gpu")
+    a = oracle.run(name, **kw)
+    b = gpu.run(name, **kw)
+    return a, b, _stats(oracle, "oracle"), _stats(gpu, "gpu")
+
+
+def assert_same(a, b, sa, sb, label):
+    assert a.color.shape == b.color.shape, label
+    dc = np.count_nonzero(a.color != b.color)
+    dd = np.count_nonzero(a.depth.view(np.uint32) != b.depth.view(np.uint32))
+    ds = np.count_nonzero(a.stencil != b.stencil)
+    assert (dc, dd, ds) == (0, 0, 0), f"{label}: differing pixels colour={dc} depth={dd} stencil={ds}"
+    assert sa[:3] == sb[:3], f"{label}: stats oracle={sa} gpu={sb} (covered, shaded, primitives)"
+
+
+@pytest.mark.parametrize("name", REFERENCE_SCENES)
+def test_reference_scene(oracle, gpu, name):
+    a, b, sa, sb = run_both(oracle, gpu, name)
+    assert_same(a, b, sa, sb, name)
+
+
+@pytest.mark.parametrize("name", REFERENCE_SCENES)
+def test_golden_image(gpu, name):
+    from PIL import Image
+    img = gpu.run(name).rgb().astype(np.int32)
+    ref = np.asarray(Image.open(os.path.join(GOLDEN, name + ".png")).convert("RGB")).astype(np.int32)
+    assert img.shape == ref.shape
+    assert int(np.abs(img - ref).max()) <= 1
+
+
+def test_cfg_tris_small(oracle, gpu):
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/tris", p0=300, seed=1)
+    assert_same(a, b, sa, sb, "cfg/tris 300")
+
+
+def test_cfg_tris_512(oracle, gpu):
+    """Random clip-space triangles (2,000, 512x512, depth LESS): heavy clipping."""
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/tris", p0=2000, seed=0x5EED0001)
+    assert_same(a, b, sa, sb, "cfg/tris 2000")
+
+
+@pytest.fixture(scope="module")
+def noise_tex(tmp_path_factory):
+    return write_noise_png(str(tmp_path_factory.mktemp("tex") / "noise1024.png"), 1024, 0x5EED0002)
+
+
+def test_cfg_mesh_small(oracle, gpu, noise_tex):
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/mesh", width=480, height=270, p0=32, p1=64, tex_path=noise_tex)
+    assert_same(a, b, sa, sb, "cfg/mesh small")
+
+
+def test_cfg_mesh_1080p(oracle, gpu, noise_tex):
+    """The indexed textured UV sphere at 1920x1080 (256x512 segments)."""
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/mesh", width=1920, height=1080, p0=256, p1=512, tex_path=noise_tex)
+    assert_same(a, b, sa, sb, "cfg/mesh 1080p")
+
+
+def test_cfg_mix_small(oracle, gpu):
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/mix", width=480, height=270, p0=20000, p1=4000, p2=4000, seed=3)
+    assert_same(a, b, sa, sb, "cfg/mix small")
+
+
+@pytest.mark.parametrize("seed", range(0, 200))
+def test_fuzz(oracle, gpu, seed):
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/fuzz", seed=seed)
+    assert_same(a, b, sa, sb, f"fuzz seed {seed}")
