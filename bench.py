@@ -1,0 +1,233 @@
+"""Benchmark of the draw pipeline on one MI355X.
+
+Workload (default): the indexed textured mesh — 1920x1080 RGBA8 colour,
+FP32 depth, 8-bit stencil; a 256x512-segment UV sphere (131,841 vertices,
+786,432 uint32 indices, 262,144 triangles); perspective MVP (fov 60 deg,
+near 0.1, far 100); 1024x1024 seeded-noise texture, nearest sampling,
+perspective-correct UV, flat per-face tag; scissor to the central 75%;
+stencil INCR on depth pass; depth LESS against a +1 depth buffer.
+
+A step is one frame: clear, depth/stencil init, the draw, until the
+colour/depth/stencil buffers are final in device memory (inputs resident
+in HBM; no host copies in the timed region).  The metric is shaded
+fragments (fragment shader invocations, identical to the reference's
+count) per second.  The CPU baseline is the oracle (a single-threaded C
+restatement of the reference's draw path) on the same frame, one core.
+
+    python bench.py [--steps K] [--warmup W] [--workload mesh|mix]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+from scene_runner import GPU_SO, ORACLE_SO, write_noise_png  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+RASTER_RECORD_BYTES = 96   # one SrpTri record (srp/device/abi.h)
+
+
+def load(path):
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    lib.srpBenchMeshSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
+    lib.srpBenchMixSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
+    lib.srpBenchRead.argtypes = [ctypes.c_void_p] * 3
+    return lib
+
+
+def stats(lib, fn):
+    a = (ctypes.c_uint64 * 4)()
+    getattr(lib, fn)(a)
+    return list(a)
+
+
+def setup(lib, args, tex):
+    if args.workload == "mesh":
+        rc = lib.srpBenchMeshSetup(args.width, args.height, args.rings, args.segs, tex.encode())
+    else:
+        rc = lib.srpBenchMixSetup(args.width, args.height, args.tris, args.lines, args.points, 0x5EED0003)
+    if rc != 0:
+        raise RuntimeError(f"bench setup failed: {rc}")
+
+
+def read_frame(lib, n):
+    c = np.zeros(n, np.uint32)
+    d = np.zeros(n, np.float32)
+    s = np.zeros(n, np.uint8)
+    lib.srpBenchRead(c.ctypes.data, d.ctypes.data, s.ctypes.data)
+    return c, d, s
+
+
+def cpu_baseline(args, tex, budget_s):
+    """Oracle frames on one host core until ~budget_s of work (at least 2)."""
+    lib = load(ORACLE_SO)
+    setup(lib, args, tex)
+    cores = None
+    try:
+        cores = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(cores)})
+    except (AttributeError, OSError):
+        pass
+    try:
+        lib.srpBenchFrame()                      # warm-up
+        lib.srpOracleResetStats()
+        t0 = time.perf_counter()
+        frames = 0
+        while frames < 2 or (time.perf_counter() - t0 < budget_s and frames < 200):
+            lib.srpBenchFrame()
+            frames += 1
+        dt = time.perf_counter() - t0
+    finally:
+        if cores:
+            os.sched_setaffinity(0, cores)
+    st = stats(lib, "srpOracleGetStats")
+    frame = read_frame(lib, args.width * args.height)
+    lib.srpBenchFinish()
+    return {
+        "value": round(st[1] / dt / 1e6, 4),
+        "unit": "shaded Mfrags/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{frames} full frames of the same workload, {dt:.1f} s, {dt / frames * 1e3:.1f} ms/frame",
+        "ms_per_frame": round(dt / frames * 1e3, 3),
+    }, frame, st[1] // frames
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=["mesh", "mix"], default="mesh")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--rings", type=int, default=256)
+    ap.add_argument("--segs", type=int, default=512)
+    ap.add_argument("--tris", type=int, default=500000)
+    ap.add_argument("--lines", type=int, default=100000)
+    ap.add_argument("--points", type=int, default=100000)
+    ap.add_argument("--tile", type=int, default=0, help="raster tile side (8 or 16); 0 = library default")
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for the CPU baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    tmp = tempfile.mkdtemp(prefix="srp_bench_")
+    tex = write_noise_png(os.path.join(tmp, "noise1024.png"), 1024, 0x5EED0002)
+
+    gpu = load(GPU_SO)
+    if args.tile:
+        gpu.srpHipSetTileSize(args.tile)
+    setup(gpu, args, tex)
+    for _ in range(args.warmup):
+        gpu.srpBenchFrame()
+    gpu.srpHipSynchronize()
+
+    gpu.srpHipResetStats()
+    gpu.srpHipProfileEnable(True)
+    gpu.srpHipSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gpu.srpBenchFrame()
+    gpu.srpHipSynchronize()
+    t1 = time.perf_counter()
+    ms = (ctypes.c_double * 4)()
+    launches = (ctypes.c_uint64 * 4)()
+    gpu.srpHipProfileRead(ms, launches)
+    gpu.srpHipProfileEnable(False)
+    st = stats(gpu, "srpHipGetStats")
+    n_px = args.width * args.height
+    gframe = read_frame(gpu, n_px)
+    gpu.srpBenchFinish()
+
+    dt = t1 - t0
+    steps = args.steps
+    shaded_per_frame = st[1] // steps
+    covered_per_frame = st[0] // steps
+    prims_per_frame = st[2] // steps
+    draws_per_frame = st[3] // steps
+    value = st[1] / dt / 1e6
+    ms_per_step = dt / steps * 1e3
+    in_tris = (args.rings * args.segs * 2) if args.workload == "mesh" else args.tris
+
+    # Dominant kernel: the triangle rasteriser.  Algorithmic bytes per launch:
+    # colour+depth+stencil read and written once (W*H*9*2), one RGBA8 texel per
+    # shaded fragment (mesh workload), one setup record per assembled triangle.
+    raster_ms = ms[0] / max(launches[0], 1)
+    tex_bytes = 4 if args.workload == "mesh" else 0
+    raster_bytes = n_px * 9 * 2 + shaded_per_frame * tex_bytes + prims_per_frame * RASTER_RECORD_BYTES
+    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9 if raster_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", f"raster_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    result = {
+        "metric": "shaded Mfrags/s @1080p (indexed textured mesh)" if args.workload == "mesh" else "shaded Mfrags/s @1080p (primitive mix)",
+        "value": round(value, 3),
+        "unit": "shaded Mfrags/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded UV sphere + noise texture)" if args.workload == "mesh" else "synthetic (seeded clip-space primitives)",
+        "config": {
+            "workload": "indexed textured UV sphere 256x512 segs, 262,144 tris, 1920x1080, scissor 75%, stencil INCR, depth LESS"
+            if args.workload == "mesh" else f"{args.tris} tris + {args.lines} lines + {args.points} points, 1920x1080, LEQUAL, cull back",
+            "width": args.width, "height": args.height,
+            "tile": int(gpu.srpHipGetTileSize()),
+        },
+        "frame_ms": round(ms_per_step, 4),
+        "triangles_per_s": round(in_tris / (ms_per_step * 1e-3), 1),
+        "shaded_per_frame": int(shaded_per_frame),
+        "covered_per_frame": int(covered_per_frame),
+        "primitives_per_frame": int(prims_per_frame),
+        "draws_per_frame": int(draws_per_frame),
+        "kernel_ms": {
+            "raster_tris": round(raster_ms, 4),
+            "draw_total": round(ms[1] / max(launches[1], 1), 4),
+            "fragment_lists": round(ms[2] / max(launches[2], 1), 4) if launches[2] else None,
+            "vertex_and_count_pass": round(ms[3] / max(launches[3], 1), 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "srp_raster_kernel",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "algorithmic_bytes": int(raster_bytes),
+        },
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        cpu, cframe, cpu_shaded = cpu_baseline(args, tex, args.cpu_budget)
+        result["cpu_baseline"] = cpu
+        same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(gframe, cframe))
+        result["parity_vs_oracle"] = bool(same and cpu_shaded == shaded_per_frame)
+        result["speedup_vs_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
+
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 3u
+#define SRP_ABI_VERSION 5u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -139,13 +139,16 @@ typedef struct SrpRasterLaunch
 {
 	const SrpDrawParams* params;
 	const SrpTri* tris;
-	const uint32_t* tileList;
+	const uint32_t* tileList;   /* pair indices, grouped by tile, in primitive order */
 	const uint32_t* tileStart;  /* nTiles + 1 */
+	const uint32_t* pairRec;    /* triangle record of a pair | 0x80000000 if pairMask is exact */
+	const unsigned long long* pairMask;  /* exact coverage of the tile's 8x8 pixels */
 	uint32_t nTiles;
 	uint32_t lateZ;
 	uint32_t words;             /* fast interpolation path usable */
 	uint32_t tile;              /* tile side: 8 or 16 */
 	uint32_t nBlocks;           /* grid size (for the XCD-aware tile order) */
+	uint32_t batched;           /* batched three-phase kernel (tile 8, one wave per tile) */
 } SrpRasterLaunch;
 
 typedef struct SrpFragLaunch

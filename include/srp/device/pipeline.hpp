@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
 
 	for (uint32_t i = begin; i < end; i++)
 	{
-		const SrpTri* T = &L.tris[L.tileList[i]];
+		const SrpTri* T = &L.tris[L.pairRec[L.tileList[i]] & 0x7FFFFFFFu];
 		if (!(px >= (int) T->x0 && px < (int) T->x1 && py >= (int) T->y0 && py < (int) T->y1))
 			continue;
 		const int ny = py - (int) T->y0, nx = px - (int) T->x0;
@@ -428,6 +428,306 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
 		P->stencil[idx] = (uint8_t) ps.stencil;
 	}
 	flushStats(P, ps.covered, ps.shaded);
+}
+
+/* Batched tile rasteriser (one 64-lane wave per 8x8 tile).  A long tile
+ * list (thousands of slivers around a mesh pole, say) would serialise one
+ * shader call, and its memory latency, per primitive.  Instead the list is
+ * consumed in batches of up to 64 primitives / RF fragments:
+ *   A  coverage: lane b holds primitive b's 64-bit mask of the tile, exact
+ *      from binning for small triangles, else found here pixel-parallel by
+ *      the reference's barycentric replay; a wave scan of the mask
+ *      popcounts places the batch's fragments in LDS in (primitive, pixel)
+ *      order;
+ *   B  shading, lane = fragment: interpolation and the fragment shader for
+ *      64 fragments at a time.  The shader is a pure function of its
+ *      inputs, so it may run before the ordered tests; a fragment is
+ *      skipped only when it is certain to fail (scissor, or an early depth
+ *      test with a monotone compare op against the depth the pixel had
+ *      when the batch began: stored depth only ever moves towards values
+ *      that fail more);
+ *   C  resolve, lane = pixel: stencil, depth and colour applied in
+ *      primitive order exactly as fragment.c:63-125 does, counting a
+ *      fragment as shaded only where the reference invokes the shader. */
+constexpr int RF = 256;
+
+enum : uint8_t { FRAG_SCISSORED = 1, FRAG_CULLED = 2, FRAG_SHADED = 3 };
+
+__device__ __forceinline__ bool monotoneOp(uint32_t op)
+{
+	return op == SRP_COMPARE_LESS || op == SRP_COMPARE_LEQUAL || op == SRP_COMPARE_GREATER ||
+	       op == SRP_COMPARE_GEQUAL || op == SRP_COMPARE_NEVER;
+}
+
+/* Barycentrics of tile pixel p (lane index layout) for triangle T. */
+__device__ __forceinline__ void baryAt(const SrpTri& T, int x, int y, float* l)
+{
+	const int ny = y - (int) T.y0, nx = x - (int) T.x0;
+#pragma unroll
+	for (int e = 0; e < 3; e++)
+		l[e] = replay(replay(T.l0[e], T.dy[e], ny), T.dx[e], nx);
+}
+
+__device__ __forceinline__ bool insideTri(const SrpTri& T, const float* l)
+{
+	bool in = true;
+#pragma unroll
+	for (int e = 0; e < 3; e++)
+	{
+		const bool tl = (T.flags >> e) & 1u;
+		in = in && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
+	}
+	return in;
+}
+
+template <class FS, bool LATE, bool WORDS>
+__global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
+{
+	__shared__ SrpTri recs[64];
+	__shared__ unsigned long long triMask[64];
+	__shared__ uint32_t fragBase[64];
+	__shared__ float fragL[3][RF];
+	__shared__ uint8_t fragPix[RF];
+	__shared__ uint8_t fragTri[RF];
+	__shared__ uint32_t fragCol[RF];
+	__shared__ float fragDep[RF];
+	__shared__ uint8_t fragState[RF];
+
+	const SrpDrawParams* P = L.params;
+	const uint32_t lane = threadIdx.x;
+	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
+	if (tile >= L.nTiles)
+		return;
+	const uint32_t begin = L.tileStart[tile], end = L.tileStart[tile + 1];
+	if (begin == end)
+		return;
+	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;
+	const int px = (int) (tx * 8 + (lane & 7u));
+	const int py = (int) (ty * 8 + (lane >> 3));
+	const bool inFb = px < (int) P->width && py < (int) P->height;
+	const size_t idx = (size_t) py * P->width + (size_t) px;
+	const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
+	const SrpState& st = P->st;
+
+	uint32_t col = 0, sten = 0, covered = 0, shaded = 0;
+	float dep = 0.0f;
+	if (inFb)
+	{
+		col = P->color[idx];
+		dep = P->depth[idx];
+		sten = P->stencil[idx];
+	}
+	const uint32_t pv = st.provokingLast ? 2u : 0u;
+	const bool precull = !LATE && st.depthTest && monotoneOp(st.depthOp);
+
+	for (uint32_t i = begin; i < end;)
+	{
+		/* A: masks.  Lane b loads pair i+b and its record. */
+		const uint32_t avail = (end - i < 64u) ? end - i : 64u;
+		uint64_t mask = 0;
+		bool known = true;
+		if (lane < avail)
+		{
+			const uint32_t pi = L.tileList[i + lane];
+			const uint32_t pr = L.pairRec[pi];
+			known = (pr >> 31) != 0;
+			mask = known ? L.pairMask[pi] : 0ull;
+			recs[lane] = L.tris[pr & 0x7FFFFFFFu];
+		}
+		__syncthreads();
+		/* Triangles without an exact mask: pixel-parallel coverage. */
+		uint64_t unknown = __ballot(lane < avail && !known);
+		while (unknown)
+		{
+			const uint32_t b = (uint32_t) __builtin_ctzll(unknown);
+			unknown &= unknown - 1ull;
+			const SrpTri& T = recs[b];
+			bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
+			if (cov)
+			{
+				float l[3];
+				baryAt(T, px, py, l);
+				cov = insideTri(T, l);
+			}
+			const uint64_t m = __ballot(cov);
+			if (lane == b)
+				mask = m;
+		}
+		/* Inclusive scan of fragment counts; keep the longest prefix that fits. */
+		const uint32_t cnt = (lane < avail) ? (uint32_t) __popcll(mask) : 0u;
+		uint32_t incl = cnt;
+		for (int off = 1; off < 64; off <<= 1)
+		{
+			const uint32_t v = __shfl_up(incl, off);
+			if ((int) lane >= off) incl += v;
+		}
+		const uint32_t nb = (uint32_t) __popcll(__ballot(lane < avail && incl <= (uint32_t) RF));
+		const uint32_t nFrag = __shfl(incl, (int) nb - 1);
+		if (lane < nb)
+		{
+			triMask[lane] = mask;
+			fragBase[lane] = incl - cnt;
+		}
+		__syncthreads();
+		/* Fragment list: exact-mask triangles write their own entries;
+		 * barycentrics of these are replayed in B. */
+		if (lane < nb && known)
+		{
+			uint64_t m = mask;
+			uint32_t pos = incl - cnt;
+			while (m)
+			{
+				const uint32_t p = (uint32_t) __builtin_ctzll(m);
+				m &= m - 1ull;
+				fragPix[pos] = (uint8_t) p;
+				fragTri[pos] = (uint8_t) lane;
+				pos++;
+			}
+		}
+		/* ...other triangles pixel-parallel, storing their barycentrics. */
+		uint64_t unk = __ballot(lane < nb && !known);
+		while (unk)
+		{
+			const uint32_t b = (uint32_t) __builtin_ctzll(unk);
+			unk &= unk - 1ull;
+			const uint64_t m = triMask[b];
+			if ((m >> lane) & 1ull)
+			{
+				const SrpTri& T = recs[b];
+				float l[3];
+				baryAt(T, px, py, l);
+				const uint32_t pos = fragBase[b] + (uint32_t) __popcll(m & ltMask);
+				fragL[0][pos] = l[0];
+				fragL[1][pos] = l[1];
+				fragL[2][pos] = l[2];
+				fragPix[pos] = (uint8_t) lane;
+				fragTri[pos] = (uint8_t) b;
+			}
+		}
+		const uint64_t knownLanes = __ballot(known);
+		__syncthreads();
+
+		/* B: shading */
+		for (uint32_t f0 = 0; f0 < nFrag; f0 += 64)
+		{
+			const uint32_t f = f0 + lane;
+			const uint32_t p = (f < nFrag) ? fragPix[f] : 0u;
+			const float pixDepth = __shfl(dep, (int) p);
+			if (f < nFrag)
+			{
+				const uint32_t b = fragTri[f];
+				const SrpTri& T = recs[b];
+				const int fx = (int) (tx * 8 + (p & 7u)), fy = (int) (ty * 8 + (p >> 3));
+				float l[3];
+				if ((knownLanes >> b) & 1ull)
+					baryAt(T, fx, fy, l);
+				else
+				{
+					l[0] = fragL[0][f];
+					l[1] = fragL[1][f];
+					l[2] = fragL[2][f];
+				}
+				const float iw[3] = {T.invW[0], T.invW[1], T.invW[2]};
+				const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
+				float depth = T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2];
+				uint8_t state = FRAG_SHADED;
+				if (st.scissorEnabled)
+				{
+					const uint64_t ux = (uint64_t) (int64_t) fx, uy = (uint64_t) (int64_t) fy;
+					if (ux < st.sx0 || ux >= st.sx1 || uy < st.sy0 || uy >= st.sy1)
+						state = FRAG_SCISSORED;
+				}
+				if (state == FRAG_SHADED && precull && !compare<float>(st.depthOp, depth, pixDepth))
+					state = FRAG_CULLED;
+				if (state == FRAG_SHADED)
+				{
+					SRPFragmentShaderIn in;
+					in.uniform = (SRPUniform*) P->uniform;
+					in.fragCoord[0] = (float) ((double) fx + 0.5);
+					in.fragCoord[1] = (float) ((double) fy + 0.5);
+					in.fragCoord[2] = depth;
+					in.fragCoord[3] = rec;
+					in.frontFacing = (T.flags >> 3) & 1u;
+					in.primitiveID = T.id;
+					SRPFragmentShaderOut out;
+					out.color[0] = 0.0f; out.color[1] = 0.0f; out.color[2] = 0.0f; out.color[3] = 0.0f;
+					out.fragDepth = __builtin_nanf("");
+					if (WORDS)
+					{
+						uint32_t buf[SRP_MAX_WORDS];
+						const uint32_t* v[3];
+#pragma unroll
+						for (int k = 0; k < 3; k++) v[k] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T.vref[k]));
+						interpWords<3>(P, v, l, iw, rec, pv, buf);
+						in.varyings = (SRPInterpolated*) buf;
+						FS()(&in, &out);
+					}
+					else
+					{
+						uint8_t buf[SRP_MAX_VARYING_BYTES];
+						const uint8_t* v[3];
+#pragma unroll
+						for (int k = 0; k < 3; k++) v[k] = varyingPtr(P, T.vref[k]);
+						for (uint32_t bb = 0; bb < P->describedSize; bb++) buf[bb] = 0;
+						interpBytes<3>(P, v, l, iw, rec, pv, buf);
+						in.varyings = (SRPInterpolated*) buf;
+						FS()(&in, &out);
+					}
+					if (LATE && !isnan(out.fragDepth))
+						depth = out.fragDepth;
+					fragCol[f] = packColor(out.color);
+				}
+				fragDep[f] = depth;
+				fragState[f] = state;
+			}
+		}
+		__syncthreads();
+
+		/* C: ordered resolve.  The pixel's covering primitives, in order. */
+		uint64_t pm = 0;
+		for (uint32_t b = 0; b < nb; b++)
+			pm |= ((triMask[b] >> lane) & 1ull) << b;
+		while (pm)
+		{
+			const uint32_t b = (uint32_t) __builtin_ctzll(pm);
+			pm &= pm - 1ull;
+			const uint32_t f = fragBase[b] + (uint32_t) __popcll(triMask[b] & ltMask);
+			covered++;
+			const uint8_t state = fragState[f];
+			if (state == FRAG_SCISSORED)
+				continue;
+			const bool front = (recs[b].flags >> 3) & 1u;
+			const SRPStencilFaceState& sf = st.stencil[front ? 0 : 1];
+			if (st.stencilEnabled && !compare<uint32_t>(sf.func, (uint32_t) (sf.ref & sf.mask), sten & sf.mask))
+			{
+				sten = stencilApply(sf.sfailOp, sten, sf);
+				continue;
+			}
+			const float d = fragDep[f];
+			if (LATE)
+				shaded++;
+			if (st.depthTest && !compare<float>(st.depthOp, d, dep))
+			{
+				if (st.stencilEnabled) sten = stencilApply(sf.dfailOp, sten, sf);
+				continue;
+			}
+			if (!LATE)
+				shaded++;
+			if (st.stencilEnabled) sten = stencilApply(sf.passOp, sten, sf);
+			col = fragCol[f];
+			if (st.depthTest && st.depthWrite)
+				dep = d;
+		}
+		__syncthreads();
+		i += nb;
+	}
+	if (inFb)
+	{
+		P->color[idx] = col;
+		P->depth[idx] = dep;
+		P->stencil[idx] = (uint8_t) sten;
+	}
+	flushStats(P, covered, shaded);
 }
 
 /* Lines and points: fragments sorted (stably) by pixel; the first thread
@@ -521,11 +821,31 @@ void launchRasterT(const SrpRasterLaunch* L, hipStream_t s)
 		hipLaunchKernelGGL((srp_raster_kernel<FS, LATE, WORDS, 8>), dim3(L->nBlocks), dim3(256), 0, s, *L);
 }
 
+template <class FS, bool LATE, bool WORDS>
+void launchBatchedT(const SrpRasterLaunch* L, hipStream_t s)
+{
+	hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+}
+
 template <class FS>
 void launchRaster(const SrpRasterLaunch* L, void* stream)
 {
 	hipStream_t s = (hipStream_t) stream;
 	if (L->nBlocks == 0) return;
+	if (L->batched)
+	{
+		if (L->lateZ)
+		{
+			if (L->words) launchBatchedT<FS, true, true>(L, s);
+			else launchBatchedT<FS, true, false>(L, s);
+		}
+		else
+		{
+			if (L->words) launchBatchedT<FS, false, true>(L, s);
+			else launchBatchedT<FS, false, false>(L, s);
+		}
+		return;
+	}
 	if (L->lateZ)
 	{
 		if (L->words) launchRasterT<FS, true, true>(L, s);

@@ -50,9 +50,11 @@ void* srpHipGetStream(void);
 /** Wait for all queued work. */
 void srpHipSynchronize(void);
 
-/** Optional per-kernel timing with HIP events on the library stream:
- *  enable, then read the accumulated milliseconds and launch counts of the
- *  triangle rasteriser (slot 0) and of the whole per-draw pipeline (slot 1). */
+/** Optional timing with HIP events on the library stream (no host sync is
+ *  added to draws): enable, then read accumulated milliseconds and counts
+ *  per slot: 0 triangle rasteriser kernel, 1 whole draw (first to last GPU
+ *  operation, including the wait for the host between the two passes),
+ *  2 line/point fragment kernel, 3 vertex stage + assembly count pass. */
 void srpHipProfileEnable(bool enable);
 void srpHipProfileRead(double ms[4], uint64_t launches[4]);
 

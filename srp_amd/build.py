@@ -50,9 +50,9 @@ def _hip_obj(src, obj, includes):
     return obj
 
 
-def _c_obj(src, obj, includes):
+def _c_obj(src, obj, includes, defines=()):
     inc = sum((["-I", d] for d in includes), [])
-    _run([CC] + C_FLAGS + inc + ["-c", src, "-o", obj])
+    _run([CC] + C_FLAGS + inc + [f"-D{d}" for d in defines] + ["-c", src, "-o", obj])
     return obj
 
 
@@ -77,7 +77,7 @@ def build_library(jobs=8):
     return LIB
 
 
-def build_shader_module(hip_sources, c_sources, out, includes=(), jobs=8):
+def build_shader_module(hip_sources, c_sources, out, includes=(), jobs=8, defines=()):
     """A shader module: HIP files with SRP_HIP_*_SHADER registrations plus
     optional host C files, linked against libsrp_amd.so."""
     objdir = os.path.join(os.path.dirname(out), "obj")
@@ -90,7 +90,7 @@ def build_shader_module(hip_sources, c_sources, out, includes=(), jobs=8):
             tasks.append(ex.submit(_hip_obj, s, o, inc))
         for s in c_sources:
             o = os.path.join(objdir, os.path.basename(s) + ".o")
-            tasks.append(ex.submit(_c_obj, s, o, inc))
+            tasks.append(ex.submit(_c_obj, s, o, inc, defines))
         objs = [t.result() for t in tasks]
     rpath = os.path.relpath(PKG, os.path.dirname(out))
     tmp = out + ".tmp"
@@ -105,8 +105,8 @@ def build_scenes(jobs=8):
     host = os.path.join(CSRC, "host")
     return build_shader_module(
         [os.path.join(SCENE_DIR, "device_shaders.hip")],
-        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c")] + [os.path.join(host, "obj_load.c")],
-        SCENE_LIB, includes=[SCENE_DIR, host], jobs=jobs)
+        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c", "bench.c")] + [os.path.join(host, "obj_load.c")],
+        SCENE_LIB, includes=[SCENE_DIR, host], jobs=jobs, defines=["SRP_GPU_BUILD"])
 
 
 def build_oracle():

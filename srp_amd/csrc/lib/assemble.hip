@@ -344,12 +344,65 @@ __device__ bool setupTriangle(const SrpDrawParams* P, Vtx v0, Vtx v1, Vtx v2, ui
 	return true;
 }
 
-__device__ __forceinline__ uint32_t tilePairs(const SrpDrawParams* P, const SrpTri& T)
+/* Tiles a triangle is binned to.  A triangle whose bounding box is at
+ * most SRP_SMALL_BBOX pixels on each side is scanned here exactly as the
+ * reference scans it (raster/triangle.c:78-110: rows from the box origin,
+ * dldy per row, dldx per pixel), giving one 64-bit coverage mask per 8x8
+ * tile; tiles it does not cover get no entry.  Larger triangles are binned
+ * to every tile of their box and their coverage is found by the rasteriser.
+ * Calls emit(tileKey, mask, known) in row-major tile order. */
+template <class F>
+__device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const SrpTri& T, F&& emit)
 {
 	if (T.x1 <= T.x0 || T.y1 <= T.y0)
 		return 0;
-	const uint32_t s = P->tileShift;
-	return (((T.x1 - 1u) >> s) - (T.x0 >> s) + 1u) * (((T.y1 - 1u) >> s) - (T.y0 >> s) + 1u);
+	const uint32_t sh = P->tileShift;
+	const uint32_t tx0 = T.x0 >> sh, tx1 = (T.x1 - 1u) >> sh;
+	const uint32_t ty0 = T.y0 >> sh, ty1 = (T.y1 - 1u) >> sh;
+	const int bw = (int) T.x1 - (int) T.x0, bh = (int) T.y1 - (int) T.y0;
+	uint32_t n = 0;
+	if (sh == 3 && bw <= SRP_SMALL_BBOX && bh <= SRP_SMALL_BBOX)
+	{
+		uint64_t mask[4][4];
+		for (int a = 0; a < 4; a++)
+			for (int b = 0; b < 4; b++) mask[a][b] = 0;
+		float row[3] = {T.l0[0], T.l0[1], T.l0[2]};
+		for (int y = T.y0; y < (int) T.y1; y++)
+		{
+			float l[3] = {row[0], row[1], row[2]};
+			for (int x = T.x0; x < (int) T.x1; x++)
+			{
+				bool inside = true;
+				for (int e = 0; e < 3; e++)
+				{
+					const bool tl = (T.flags >> e) & 1u;
+					inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
+				}
+				if (inside)
+					mask[(y >> 3) - (int) ty0][(x >> 3) - (int) tx0] |= 1ull << (((y & 7) << 3) | (x & 7));
+				for (int e = 0; e < 3; e++) l[e] += T.dx[e];
+			}
+			for (int e = 0; e < 3; e++) row[e] += T.dy[e];
+		}
+		for (uint32_t ty = ty0; ty <= ty1; ty++)
+			for (uint32_t tx = tx0; tx <= tx1; tx++)
+			{
+				const uint64_t m = mask[ty - ty0][tx - tx0];
+				if (m)
+				{
+					emit(ty * P->tilesX + tx, m, true);
+					n++;
+				}
+			}
+		return n;
+	}
+	for (uint32_t ty = ty0; ty <= ty1; ty++)
+		for (uint32_t tx = tx0; tx <= tx1; tx++)
+		{
+			emit(ty * P->tilesX + tx, ~0ull, false);
+			n++;
+		}
+	return n;
 }
 
 /* line.c:79-86 */
@@ -551,26 +604,23 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 			SrpTri T;
 			if (!setupTriangle(P, a, b, c, recBase + nRec, T))
 				continue;
-			const uint32_t np = tilePairs(P, T);
+			const uint32_t r = recBase + nRec;
+			uint32_t np;
 			if (EMIT)
 			{
-				const uint32_t r = recBase + nRec;
 				A.tris[r] = T;
-				if (np)
-				{
-					const uint32_t sh = P->tileShift;
-					uint32_t o = itemBase + nItems;
-					for (uint32_t ty = T.y0 >> sh; ty <= (T.y1 - 1u) >> sh; ty++)
-						for (uint32_t tx = T.x0 >> sh; tx <= (T.x1 - 1u) >> sh; tx++)
-						{
-							const uint32_t key = ty * P->tilesX + tx;
-							A.pairKeys[o] = key;
-							A.pairVals[o] = r;
-							atomicAdd(&A.tileCount[key], 1u);
-							o++;
-						}
-				}
+				uint32_t o = itemBase + nItems;
+				np = binTriangle(P, T, [&](uint32_t key, uint64_t m, bool known) {
+					A.pairKeys[o] = key;
+					A.pairVals[o] = o;
+					A.pairRec[o] = r | (known ? 0x80000000u : 0u);
+					A.pairMask[o] = m;
+					atomicAdd(&A.tileCount[key], 1u);
+					o++;
+				});
 			}
+			else
+				np = binTriangle(P, T, [](uint32_t, uint64_t, bool) {});
 			nRec++;
 			nItems += np;
 		}

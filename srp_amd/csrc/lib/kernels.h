@@ -10,6 +10,7 @@
 
 #define SRP_POOL 32       /* clip pool: 3 originals + generated vertices */
 #define SRP_MAX_POLY 16   /* vertices of a clipped polygon */
+#define SRP_SMALL_BBOX 24 /* triangles up to this box side get exact tile masks when binned */
 
 namespace srp {
 namespace lib {
@@ -32,8 +33,10 @@ struct AsmArgs
 	SrpTri* tris;
 	SrpLine* lines;
 	SrpPoint* points;
-	uint32_t* pairKeys;
-	uint32_t* pairVals;
+	uint32_t* pairKeys;       /* tile of each (tile, triangle) pair */
+	uint32_t* pairVals;       /* pair index (sorted along with the keys) */
+	uint32_t* pairRec;        /* triangle record | 0x80000000 when the mask is exact */
+	unsigned long long* pairMask; /* coverage of the tile's 64 pixels */
 	SrpFrag* frags;
 	uint32_t* fragKeys;
 	uint32_t* tileCount;

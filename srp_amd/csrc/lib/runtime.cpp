@@ -101,20 +101,49 @@ struct Runtime
 	unsigned long long* dStats = nullptr;
 	uint64_t hostPrims = 0, hostDraws = 0;
 	int tile = 8;
+	int batched = 1;  /* three-phase batched rasteriser (tile 8) */
 
 	DevBuf uniform, clip, varyings, genVaryings, counts, totals;
-	DevBuf tris, lines, points, pairKeys, pairVals, frags, fragKeys, order, tileCount;
+	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount;
 	ScanScratch scan;
 	SortScratch sort;
 
 	std::unordered_map<const void*, const SrpVsEntry*> vs;
 	std::unordered_map<const void*, const SrpFsEntry*> fs;
 
+	/* Profiling: event pairs recorded around launches, read back lazily so
+	 * that timing adds no host synchronisation to the draws. */
+	struct ProfRec { hipEvent_t a, b; int slot; };
 	bool profile = false;
 	double profMs[4] = {0, 0, 0, 0};
 	uint64_t profLaunches[4] = {0, 0, 0, 0};
-	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+	std::vector<ProfRec> profPending;
+	std::vector<hipEvent_t> eventPool;
 	hipEvent_t paramsCopied = nullptr;  /* staging block free again */
+
+	hipEvent_t mark()
+	{
+		hipEvent_t e;
+		if (eventPool.empty()) HIP_CHECK(hipEventCreate(&e));
+		else { e = eventPool.back(); eventPool.pop_back(); }
+		HIP_CHECK(hipEventRecord(e, stream));
+		return e;
+	}
+	void span(int slot, hipEvent_t a, hipEvent_t b) { profPending.push_back({a, b, slot}); }
+	void collect()
+	{
+		HIP_CHECK(hipStreamSynchronize(stream));
+		for (const ProfRec& r : profPending)
+		{
+			float ms = 0.0f;
+			HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+			profMs[r.slot] += ms;
+			profLaunches[r.slot]++;
+			eventPool.push_back(r.a);
+			eventPool.push_back(r.b);
+		}
+		profPending.clear();
+	}
 
 	Runtime()
 	{
@@ -125,10 +154,11 @@ struct Runtime
 		HIP_CHECK(hipHostMalloc((void**) &hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dStats, 4 * sizeof(unsigned long long)));
 		HIP_CHECK(hipMemset(dStats, 0, 4 * sizeof(unsigned long long)));
-		for (int i = 0; i < 4; i++) HIP_CHECK(hipEventCreate(&ev[i]));
 		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
 		const char* t = getenv("SRP_TILE");
 		if (t && (atoi(t) == 8 || atoi(t) == 16)) tile = atoi(t);
+		const char* b = getenv("SRP_RASTER");
+		if (b && strcmp(b, "simple") == 0) batched = 0;
 	}
 };
 
@@ -739,6 +769,7 @@ void srpHipSynchronize(void) { HIP_CHECK(hipStreamSynchronize(rt().stream)); }
 void srpHipProfileEnable(bool enable)
 {
 	Runtime& R = rt();
+	R.collect();
 	R.profile = enable;
 	for (int i = 0; i < 4; i++) { R.profMs[i] = 0; R.profLaunches[i] = 0; }
 }
@@ -746,7 +777,7 @@ void srpHipProfileEnable(bool enable)
 void srpHipProfileRead(double ms[4], uint64_t launches[4])
 {
 	Runtime& R = rt();
-	HIP_CHECK(hipStreamSynchronize(R.stream));
+	R.collect();
 	for (int i = 0; i < 4; i++) { ms[i] = R.profMs[i]; launches[i] = R.profLaunches[i]; }
 }
 
@@ -999,7 +1030,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 
 	if (!F->deferred)
 		fbUpload(F);
-	if (R.profile) HIP_CHECK(hipEventRecord(R.ev[2], R.stream));
+	hipEvent_t evDraw = R.profile ? R.mark() : nullptr;
 	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
 
 	/* Vertex stage. */
@@ -1029,6 +1060,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	scanExclusive(A.recOff, nIn, totals + 0, R.scan, R.stream);
 	scanExclusive(A.genOff, nIn, totals + 1, R.scan, R.stream);
 	scanExclusive(A.itemOff, nIn, totals + 2, R.scan, R.stream);
+	if (R.profile) R.span(3, evDraw, R.mark());
 	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 	const uint32_t nRec = R.hTotals[0], nGen = R.hTotals[1], nItems = R.hTotals[2];
@@ -1053,6 +1085,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		A.tris = R.tris.get<SrpTri>((size_t) nRec * sizeof(SrpTri));
 		A.pairKeys = R.pairKeys.get<uint32_t>((size_t) nItems * 4 + 16);
 		A.pairVals = R.pairVals.get<uint32_t>((size_t) nItems * 4 + 16);
+		A.pairRec = R.pairRec.get<uint32_t>((size_t) nItems * 4 + 16);
+		A.pairMask = R.pairMask.get<unsigned long long>((size_t) nItems * 8 + 16);
 		tileCount = R.tileCount.get<uint32_t>(((size_t) nTiles + 2) * 4);
 		fillU32(tileCount, 0u, (size_t) nTiles + 1, R.stream);
 		A.tileCount = tileCount;
@@ -1076,15 +1110,18 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		rl.tris = A.tris;
 		rl.tileList = vals;
 		rl.tileStart = tileCount;
+		rl.pairRec = A.pairRec;
+		rl.pairMask = A.pairMask;
 		rl.nTiles = nTiles;
 		rl.lateZ = S.lateZ;
 		rl.words = H.wordPath;
 		rl.tile = (uint32_t) R.tile;
-		const uint32_t perBlock = (R.tile == 8) ? 4u : 1u;
+		rl.batched = (R.batched && R.tile == 8) ? 1u : 0u;
+		const uint32_t perBlock = rl.batched ? 1u : ((R.tile == 8) ? 4u : 1u);
 		rl.nBlocks = (nTiles + perBlock - 1) / perBlock;
-		if (R.profile) HIP_CHECK(hipEventRecord(R.ev[0], R.stream));
+		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
 		fse->raster(&rl, (void*) R.stream);
-		if (R.profile) HIP_CHECK(hipEventRecord(R.ev[1], R.stream));
+		if (R.profile) R.span(0, e0, R.mark());
 	}
 	else
 	{
@@ -1103,23 +1140,12 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		fl.kind = (backend == 1) ? 0u : 1u;
 		fl.lateZ = S.lateZ;
 		fl.words = H.wordPath;
+		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
 		fse->frags(&fl, (void*) R.stream);
+		if (R.profile) R.span(2, e0, R.mark());
 	}
 	if (R.profile)
-	{
-		HIP_CHECK(hipEventRecord(R.ev[3], R.stream));
-		HIP_CHECK(hipEventSynchronize(R.ev[3]));
-		float ms = 0.0f;
-		if (backend == 0)
-		{
-			HIP_CHECK(hipEventElapsedTime(&ms, R.ev[0], R.ev[1]));
-			R.profMs[0] += ms;
-			R.profLaunches[0]++;
-		}
-		HIP_CHECK(hipEventElapsedTime(&ms, R.ev[2], R.ev[3]));
-		R.profMs[1] += ms;
-		R.profLaunches[1]++;
-	}
+		R.span(1, evDraw, R.mark());
 	if (!F->deferred)
 		fbDownload(F);
 }
