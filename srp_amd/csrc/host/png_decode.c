@@ -33,7 +33,7 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 {
 	static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
 	FILE* f = fopen(path, "rb");
-	if (!f) return fail(err, "cannot open file", NULL, NULL);
+	if (!f) return fail(err, "can't fopen", NULL, NULL);
 	fseek(f, 0, SEEK_END);
 	const long fsize = ftell(f);
 	fseek(f, 0, SEEK_SET);

@@ -1,0 +1,60 @@
+"""Writes tests/golden/reference_hashes.json from the reference renderer
+compiled in place (oracle/_ref/libscenes_ref.so, built by `make -C oracle
+ref` from /root/reference).  For each case: SHA-256 of the colour, depth and
+stencil buffers the reference produces.  These pin the oracle (CPU tests)
+and the GPU path (GPU tests) without the reference itself being present.
+
+    python tests/golden/make_fixtures.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from scene_runner import REF_SO, REFERENCE_SCENES, SceneModule, write_noise_png  # noqa: E402
+
+NOISE_SEED = 0x5EED0002
+
+
+def cases(tex):
+    out = [dict(name=n) for n in REFERENCE_SCENES]
+    out += [
+        dict(name="cfg/tris", p0=300, seed=1),
+        dict(name="cfg/tris", p0=2000, seed=0x5EED0001),
+        dict(name="cfg/mesh", width=480, height=270, p0=32, p1=64, tex_path=tex),
+        dict(name="cfg/mesh", width=1920, height=1080, p0=256, p1=512, tex_path=tex),
+        dict(name="cfg/mix", width=480, height=270, p0=20000, p1=4000, p2=4000, seed=3),
+    ]
+    out += [dict(name="cfg/fuzz", seed=s) for s in range(200)]
+    return out
+
+
+def case_id(c):
+    return ",".join(f"{k}={v}" for k, v in sorted(c.items()) if k != "tex_path")
+
+
+def digest(frame):
+    return {
+        "color": hashlib.sha256(frame.color.tobytes()).hexdigest(),
+        "depth": hashlib.sha256(frame.depth.tobytes()).hexdigest(),
+        "stencil": hashlib.sha256(frame.stencil.tobytes()).hexdigest(),
+        "shape": list(frame.color.shape),
+    }
+
+
+def main():
+    ref = SceneModule(REF_SO)
+    tex = write_noise_png("/tmp/srp_fixture_noise1024.png", 1024, NOISE_SEED)
+    table = {}
+    for c in cases(tex):
+        table[case_id(c)] = digest(ref.run(**c))
+    with open(os.path.join(HERE, "reference_hashes.json"), "w") as f:
+        json.dump({"noise_seed": NOISE_SEED, "cases": table}, f, indent=1, sort_keys=True)
+    print(f"wrote {len(table)} cases")
+
+
+if __name__ == "__main__":
+    main()

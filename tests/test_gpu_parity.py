@@ -96,3 +96,82 @@ def test_cfg_mix_small(oracle, gpu):
 def test_fuzz(oracle, gpu, seed):
     a, b, sa, sb = run_both(oracle, gpu, "cfg/fuzz", seed=seed)
     assert_same(a, b, sa, sb, f"fuzz seed {seed}")
+
+
+# ---- against the compiled reference's fixtures (no oracle involved) ----
+
+import ctypes  # noqa: E402
+import hashlib  # noqa: E402
+import json  # noqa: E402
+
+from scene_runner import GPU_SO, ORACLE_SO, RES_DIR  # noqa: E402
+
+HASHES = json.load(open(os.path.join(REPO, "tests", "golden", "reference_hashes.json")))
+
+
+def _case_id(c):
+    return ",".join(f"{k}={v}" for k, v in sorted(c.items()) if k != "tex_path")
+
+
+def _digest(frame):
+    return {
+        "color": hashlib.sha256(frame.color.tobytes()).hexdigest(),
+        "depth": hashlib.sha256(frame.depth.tobytes()).hexdigest(),
+        "stencil": hashlib.sha256(frame.stencil.tobytes()).hexdigest(),
+        "shape": list(frame.color.shape),
+    }
+
+
+@pytest.fixture(scope="module")
+def fixture_tex(tmp_path_factory):
+    return write_noise_png(str(tmp_path_factory.mktemp("ftex") / "noise1024.png"), 1024, HASHES["noise_seed"])
+
+
+GPU_FIXTURE_CASES = [dict(name=n) for n in REFERENCE_SCENES] + [
+    dict(name="cfg/tris", p0=300, seed=1),
+    dict(name="cfg/tris", p0=2000, seed=0x5EED0001),
+    dict(name="cfg/mesh", width=480, height=270, p0=32, p1=64),
+    dict(name="cfg/mesh", width=1920, height=1080, p0=256, p1=512),
+    dict(name="cfg/mix", width=480, height=270, p0=20000, p1=4000, p2=4000, seed=3),
+] + [dict(name="cfg/fuzz", seed=s) for s in range(200)]
+
+
+@pytest.mark.parametrize("case", GPU_FIXTURE_CASES, ids=_case_id)
+def test_gpu_vs_reference_fixture(gpu, fixture_tex, case):
+    c = dict(case)
+    if c["name"] == "cfg/mesh":
+        c["tex_path"] = fixture_tex
+    print(f"[fixture] {_case_id(case)}", flush=True)
+    assert _digest(gpu.run(**c)) == HASHES["cases"][_case_id(case)]
+
+
+def _transcript(path):
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib.srpSceneMessages(RES_DIR.encode(), buf, 1 << 16)
+    return buf.value.decode()
+
+
+def test_gpu_messages_match_oracle(gpu):
+    """Error/warning callbacks (OOB draws, excess vertices, unknown
+    primitive, texture load failure, bad texture parameters)."""
+    assert _transcript(GPU_SO) == _transcript(ORACLE_SO)
+
+
+def test_deferred_frame_matches_oracle(fixture_tex):
+    """Device-resident (deferred) framebuffer with device-side depth and
+    stencil fills, as the benchmark runs it."""
+    n = 480 * 270
+    frames = []
+    for path in (GPU_SO, ORACLE_SO):
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        lib.srpBenchMeshSetup.argtypes = [ctypes.c_int] * 4 + [ctypes.c_char_p]
+        assert lib.srpBenchMeshSetup(480, 270, 64, 128, fixture_tex.encode()) == 0
+        for _ in range(3):
+            lib.srpBenchFrame()
+        c, d, s = np.zeros(n, np.uint32), np.zeros(n, np.float32), np.zeros(n, np.uint8)
+        lib.srpBenchRead(ctypes.c_void_p(c.ctypes.data), ctypes.c_void_p(d.ctypes.data), ctypes.c_void_p(s.ctypes.data))
+        lib.srpBenchFinish()
+        frames.append((c, d.view(np.uint32), s))
+    for a, b in zip(*frames):
+        assert np.array_equal(a, b)
