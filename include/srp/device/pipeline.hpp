@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks) * TILES_PER_BLOCK + ((TILE == 8) ? (t >> 6) : 0u);
 	if (tile >= L.nTiles)
 		return;
-	const uint32_t begin = L.tileStart[tile], end = L.tileStart[tile + 1];
+	const uint32_t begin = L.tileStart[tile], end = L.tileEnd[tile];
 	if (begin == end)
 		return;
 	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
 	if (tile >= L.nTiles)
 		return;
-	const uint32_t begin = L.tileStart[tile], end = L.tileStart[tile + 1];
+	const uint32_t begin = L.tileStart[tile], end = L.tileEnd[tile];
 	if (begin == end)
 		return;
 	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;

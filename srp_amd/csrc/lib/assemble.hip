@@ -540,62 +540,18 @@ __device__ __forceinline__ void triStream(uint32_t prim, uint64_t base, uint64_t
 	else { s[0] = base; s[1] = base + k + 1; s[2] = base + k + 2; }
 }
 
-/* Triangle draws (all polygon modes): primitive_assembly.c:37-135. */
-template <bool EMIT>
-__global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
+/* Assembled primitives of one clipped polygon v[0..n-1] (fan from v[0],
+ * clipping.c:107-116), per polygon mode (primitive_assembly.c:80-128). */
+template <bool EMIT, int NMAX>
+__device__ __forceinline__ void emitFan(const AsmArgs& A, const SrpDrawParams* P, const Vtx* v, int n,
+                                        uint32_t recBase, uint32_t itemBase, uint32_t& nRec, uint32_t& nItems)
 {
-	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
-	if (k >= A.nIn)
-		return;
-	const SrpDrawParams* P = A.P;
-	Pool pl;
-	uint64_t s[3];
-	triStream(A.prim, A.start, k, s);
-	for (int i = 0; i < 3; i++)
-	{
-		pl.slot[i] = slotOf(A, s[i]);
-		loadClip(P, pl.slot[i], pl.c[i]);
-	}
-	pl.n = 3;
-	uint8_t poly[SRP_MAX_POLY];
-	int n = clipTriangle(pl, poly);
-	if (n < 0)
-	{
-		atomicOr(A.error, 1u);
-		n = 0;
-	}
-
-	uint32_t recBase = 0, genBase = 0, itemBase = 0;
-	if (EMIT)
-	{
-		recBase = A.recOff[k];
-		genBase = A.genOff[k];
-		itemBase = A.itemOff[k];
-	}
-
-	/* Divide each polygon vertex once; number the generated ones. */
-	Vtx v[SRP_MAX_POLY];
-	uint32_t nGen = 0;
-	for (int i = 0; i < n; i++)
-	{
-		const int q = poly[i];
-		for (int c = 0; c < 4; c++) v[i].c[c] = pl.c[q][c];
-		v[i].invW = divide(v[i].c);
-		if (q < 3)
-			v[i].vref = pl.slot[q];
-		else
-		{
-			v[i].vref = P->cacheSize + genBase + nGen;
-			if (EMIT)
-				materialize(P, pl, q, A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
-			nGen++;
-		}
-	}
-
-	uint32_t nRec = 0, nItems = 0;
 	const uint32_t mode = P->st.polygonMode;
-	for (int f = 1; f + 1 < n; f++)
+#pragma unroll
+	for (int f = 1; f + 1 < NMAX; f++)
 	{
+		if (f + 1 >= n)
+			break;
 		const Vtx& a = v[0];
 		const Vtx& b = v[f];
 		const Vtx& c = v[f + 1];
@@ -615,7 +571,6 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 					A.pairVals[o] = o;
 					A.pairRec[o] = r | (known ? 0x80000000u : 0u);
 					A.pairMask[o] = m;
-					atomicAdd(&A.tileCount[key], 1u);
 					o++;
 				});
 			}
@@ -654,6 +609,88 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 			}
 		}
 	}
+}
+
+/* A triangle that needs clipping: Sutherland-Hodgman over the pool, then
+ * the varyings of the generated vertices that survive. */
+template <bool EMIT>
+__device__ __noinline__ void assembleClipped(const AsmArgs& A, const uint32_t* slot, uint32_t genBase,
+                                             uint32_t recBase, uint32_t itemBase,
+                                             uint32_t& nRec, uint32_t& nGen, uint32_t& nItems)
+{
+	const SrpDrawParams* P = A.P;
+	Pool pl;
+	for (int i = 0; i < 3; i++)
+	{
+		pl.slot[i] = slot[i];
+		loadClip(P, slot[i], pl.c[i]);
+	}
+	pl.n = 3;
+	uint8_t poly[SRP_MAX_POLY];
+	int n = clipTriangle(pl, poly);
+	if (n < 0)
+	{
+		atomicOr(A.error, 1u);
+		n = 0;
+	}
+	Vtx v[SRP_MAX_POLY];
+	for (int i = 0; i < n; i++)
+	{
+		const int q = poly[i];
+		for (int c = 0; c < 4; c++) v[i].c[c] = pl.c[q][c];
+		v[i].invW = divide(v[i].c);
+		if (q < 3)
+			v[i].vref = pl.slot[q];
+		else
+		{
+			v[i].vref = P->cacheSize + genBase + nGen;
+			if (EMIT)
+				materialize(P, pl, q, A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
+			nGen++;
+		}
+	}
+	emitFan<EMIT, SRP_MAX_POLY>(A, P, v, n, recBase, itemBase, nRec, nItems);
+}
+
+/* Triangle draws (all polygon modes): primitive_assembly.c:37-135.  The
+ * common case — all three vertices inside the frustum, trivially accepted
+ * by clipping.c:74-78 — stays in registers. */
+template <bool EMIT>
+__global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
+{
+	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
+	if (k >= A.nIn)
+		return;
+	const SrpDrawParams* P = A.P;
+	uint64_t s[3];
+	triStream(A.prim, A.start, k, s);
+	uint32_t slot[3];
+	Vtx v[3];
+	uint32_t codeOr = 0, codeAnd = 0x3Fu;
+	for (int i = 0; i < 3; i++)
+	{
+		slot[i] = slotOf(A, s[i]);
+		loadClip(P, slot[i], v[i].c);
+		const uint32_t c = outcode(v[i].c);
+		codeOr |= c;
+		codeAnd &= c;
+		v[i].vref = slot[i];
+	}
+	uint32_t recBase = 0, genBase = 0, itemBase = 0;
+	if (EMIT)
+	{
+		recBase = A.recOff[k];
+		genBase = A.genOff[k];
+		itemBase = A.itemOff[k];
+	}
+	uint32_t nRec = 0, nGen = 0, nItems = 0;
+	if (codeOr == 0)
+	{
+		for (int i = 0; i < 3; i++) v[i].invW = divide(v[i].c);
+		emitFan<EMIT, 3>(A, P, v, 3, recBase, itemBase, nRec, nItems);
+	}
+	else if (codeAnd == 0)
+		assembleClipped<EMIT>(A, slot, genBase, recBase, itemBase, nRec, nGen, nItems);
 	if (!EMIT)
 	{
 		A.recOff[k] = nRec;

@@ -39,7 +39,6 @@ struct AsmArgs
 	unsigned long long* pairMask; /* coverage of the tile's 64 pixels */
 	SrpFrag* frags;
 	uint32_t* fragKeys;
-	uint32_t* tileCount;
 	uint8_t* genVaryings;
 	uint32_t* error;
 };
@@ -55,6 +54,11 @@ struct ScanScratch
 };
 /* Exclusive prefix sum of n u32 values in place; *total (device) = sum. */
 void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scratch, hipStream_t s);
+/* The same for nArrays arrays of n values `stride` apart; totals[a] = sums. */
+void scanExclusiveMulti(uint32_t* base, size_t stride, int nArrays, size_t n, uint32_t* totals,
+                        ScanScratch& scratch, hipStream_t s);
+/* [start, end) of each tile's run in a tile-sorted key array. */
+void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s);
 
 struct SortScratch
 {
@@ -73,6 +77,8 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void iota(uint32_t* p, size_t n, hipStream_t s);
+void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv, uint8_t* stencil, uint8_t sv, size_t n,
+                     hipStream_t s);
 
 }  // namespace lib
 }  // namespace srp

@@ -90,6 +90,11 @@ struct DevBuf
 	}
 };
 
+/* Parameter block: SrpDrawParams, then the uniform snapshot. */
+constexpr size_t kParamsHead = (sizeof(SrpDrawParams) + 255) & ~(size_t) 255;
+constexpr size_t kMaxInlineUniform = 16384;
+constexpr size_t kParamsBlock = kParamsHead + kMaxInlineUniform;
+
 struct Runtime
 {
 	hipStream_t stream = nullptr;
@@ -149,8 +154,8 @@ struct Runtime
 	{
 		HIP_CHECK(hipSetDevice(0));
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-		HIP_CHECK(hipHostMalloc((void**) &hParams, sizeof(SrpDrawParams), hipHostMallocDefault));
-		HIP_CHECK(hipMalloc((void**) &dParams, sizeof(SrpDrawParams)));
+		HIP_CHECK(hipHostMalloc((void**) &hParams, kParamsBlock, hipHostMallocDefault));
+		HIP_CHECK(hipMalloc((void**) &dParams, kParamsBlock));
 		HIP_CHECK(hipHostMalloc((void**) &hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dStats, 4 * sizeof(unsigned long long)));
 		HIP_CHECK(hipMemset(dStats, 0, 4 * sizeof(unsigned long long)));
@@ -191,13 +196,29 @@ struct FbImpl
 	float* dDepth = nullptr;
 	uint8_t* dStencil = nullptr;
 	bool deferred = false;
+	/* Deferred mode: clears and fills are recorded and run as one fused
+	 * kernel before the next use of the device copy. */
+	bool pendColor = false, pendDepth = false, pendStencil = false;
+	uint32_t fillColor = 0, fillDepthBits = 0;
+	uint8_t fillStencil = 0;
 };
 
 FbImpl* fbImpl(const SRPFramebuffer* fb) { return (FbImpl*) fb; }
 
+void fbFlushFills(FbImpl* F)
+{
+	if (!(F->pendColor || F->pendDepth || F->pendStencil))
+		return;
+	fillFramebuffer(F->pendColor ? F->dColor : nullptr, F->fillColor,
+	                F->pendDepth ? (uint32_t*) F->dDepth : nullptr, F->fillDepthBits,
+	                F->pendStencil ? F->dStencil : nullptr, F->fillStencil, F->pub.size, rt().stream);
+	F->pendColor = F->pendDepth = F->pendStencil = false;
+}
+
 void fbUpload(FbImpl* F)
 {
 	Runtime& R = rt();
+	F->pendColor = F->pendDepth = F->pendStencil = false;  /* the host arrays win */
 	const size_t n = F->pub.size;
 	HIP_CHECK(hipMemcpyAsync(F->dColor, F->pub.color, n * 4, hipMemcpyHostToDevice, R.stream));
 	HIP_CHECK(hipMemcpyAsync(F->dDepth, F->pub.depth, n * 4, hipMemcpyHostToDevice, R.stream));
@@ -207,6 +228,7 @@ void fbUpload(FbImpl* F)
 void fbDownload(FbImpl* F)
 {
 	Runtime& R = rt();
+	fbFlushFills(F);
 	const size_t n = F->pub.size;
 	HIP_CHECK(hipMemcpyAsync(F->pub.color, F->dColor, n * 4, hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipMemcpyAsync(F->pub.depth, F->dDepth, n * 4, hipMemcpyDeviceToHost, R.stream));
@@ -562,12 +584,11 @@ void srpFramebufferClear(const SRPFramebuffer* fb)
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 	{
-		Runtime& R = rt();
-		fillU32(F->dColor, 0u, fb->size, R.stream);
 		const float m1 = -1.0f;
-		uint32_t bits;
-		memcpy(&bits, &m1, 4);
-		fillU32((uint32_t*) F->dDepth, bits, fb->size, R.stream);
+		F->pendColor = true;
+		F->fillColor = 0u;
+		F->pendDepth = true;
+		memcpy(&F->fillDepthBits, &m1, 4);
 		return;
 	}
 	memset(fb->color, 0, fb->size * 4);
@@ -597,9 +618,8 @@ void srpFramebufferFillDepth(const SRPFramebuffer* fb, float v)
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 	{
-		uint32_t bits;
-		memcpy(&bits, &v, 4);
-		fillU32((uint32_t*) F->dDepth, bits, fb->size, rt().stream);
+		F->pendDepth = true;
+		memcpy(&F->fillDepthBits, &v, 4);
 		return;
 	}
 	for (size_t i = 0; i < fb->size; i++) fb->depth[i] = v;
@@ -610,7 +630,8 @@ void srpFramebufferFillStencil(const SRPFramebuffer* fb, uint8_t v)
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 	{
-		fillU8(F->dStencil, v, fb->size, rt().stream);
+		F->pendStencil = true;
+		F->fillStencil = v;
 		return;
 	}
 	memset(fb->stencil, v, fb->size);
@@ -1000,18 +1021,28 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.stencil = F->dStencil;
 
 	const size_t usize = vse->uniformSize > fse->uniformSize ? vse->uniformSize : fse->uniformSize;
+	size_t paramsBytes = sizeof(SrpDrawParams);
 	if (sp->uniform && usize)
 	{
-		if (usize > R.uniformCap)
+		if (usize <= kMaxInlineUniform)
 		{
-			if (R.hUniform) HIP_CHECK(hipHostFree(R.hUniform));
-			R.uniformCap = usize * 2;
-			HIP_CHECK(hipHostMalloc((void**) &R.hUniform, R.uniformCap, hipHostMallocDefault));
+			memcpy((uint8_t*) R.hParams + kParamsHead, sp->uniform, usize);
+			H.uniform = (const uint8_t*) R.dParams + kParamsHead;
+			paramsBytes = kParamsHead + usize;
 		}
-		memcpy(R.hUniform, sp->uniform, usize);
-		uint8_t* dU = R.uniform.get<uint8_t>(usize + 16);
-		HIP_CHECK(hipMemcpyAsync(dU, R.hUniform, usize, hipMemcpyHostToDevice, R.stream));
-		H.uniform = dU;
+		else
+		{
+			if (usize > R.uniformCap)
+			{
+				if (R.hUniform) HIP_CHECK(hipHostFree(R.hUniform));
+				R.uniformCap = usize * 2;
+				HIP_CHECK(hipHostMalloc((void**) &R.hUniform, R.uniformCap, hipHostMallocDefault));
+			}
+			memcpy(R.hUniform, sp->uniform, usize);
+			uint8_t* dU = R.uniform.get<uint8_t>(usize + 16);
+			HIP_CHECK(hipMemcpyAsync(dU, R.hUniform, usize, hipMemcpyHostToDevice, R.stream));
+			H.uniform = dU;
+		}
 	}
 	else
 		H.uniform = nullptr;
@@ -1030,8 +1061,10 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 
 	if (!F->deferred)
 		fbUpload(F);
+	else
+		fbFlushFills(F);
 	hipEvent_t evDraw = R.profile ? R.mark() : nullptr;
-	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, paramsBytes, hipMemcpyHostToDevice, R.stream));
 
 	/* Vertex stage. */
 	SrpVsLaunch vl;
@@ -1057,9 +1090,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	A.itemOff = counts + 2 * (nIn + 1);
 	A.error = totals + 3;
 	launchAssemble(A, kind, false, R.stream);
-	scanExclusive(A.recOff, nIn, totals + 0, R.scan, R.stream);
-	scanExclusive(A.genOff, nIn, totals + 1, R.scan, R.stream);
-	scanExclusive(A.itemOff, nIn, totals + 2, R.scan, R.stream);
+	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream);
 	if (R.profile) R.span(3, evDraw, R.mark());
 	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
@@ -1079,7 +1110,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
 	A.genVaryings = H.genVaryings;
 	const uint32_t nTiles = H.tilesX * H.tilesY;
-	uint32_t* tileCount = nullptr;
+	uint32_t* tileStart = nullptr;
+	uint32_t* tileEnd = nullptr;
 	if (backend == 0)
 	{
 		A.tris = R.tris.get<SrpTri>((size_t) nRec * sizeof(SrpTri));
@@ -1087,9 +1119,9 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		A.pairVals = R.pairVals.get<uint32_t>((size_t) nItems * 4 + 16);
 		A.pairRec = R.pairRec.get<uint32_t>((size_t) nItems * 4 + 16);
 		A.pairMask = R.pairMask.get<unsigned long long>((size_t) nItems * 8 + 16);
-		tileCount = R.tileCount.get<uint32_t>(((size_t) nTiles + 2) * 4);
-		fillU32(tileCount, 0u, (size_t) nTiles + 1, R.stream);
-		A.tileCount = tileCount;
+		tileStart = R.tileCount.get<uint32_t>(((size_t) 2 * nTiles + 2) * 4);
+		tileEnd = tileStart + nTiles;
+		fillU32(tileStart, 0u, (size_t) 2 * nTiles, R.stream);
 	}
 	else
 	{
@@ -1104,12 +1136,13 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	{
 		uint32_t *keys, *vals;
 		radixSort(A.pairKeys, A.pairVals, nItems, bitsFor(nTiles), R.sort, &keys, &vals, R.stream);
-		scanExclusive(tileCount, (size_t) nTiles + 1, totals + 4, R.scan, R.stream);
+		tileBounds(keys, nItems, tileStart, tileEnd, R.stream);
 		SrpRasterLaunch rl;
 		rl.params = R.dParams;
 		rl.tris = A.tris;
 		rl.tileList = vals;
-		rl.tileStart = tileCount;
+		rl.tileStart = tileStart;
+		rl.tileEnd = tileEnd;
 		rl.pairRec = A.pairRec;
 		rl.pairMask = A.pairMask;
 		rl.nTiles = nTiles;

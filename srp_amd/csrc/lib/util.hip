@@ -92,6 +92,119 @@ void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scrat
 	scanLevel(data, n, total, scratch.buf, s);
 }
 
+/* Two-launch exclusive scan of up to a few equal-length arrays (grid.y
+ * selects the array): block-local scans and block totals, then each block
+ * adds the sum of the totals before it (summed redundantly per block, which
+ * is cheaper than a third launch while there are at most a few thousand
+ * blocks).  The last block writes the array's grand total. */
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(uint32_t* base, size_t stride, size_t n, uint32_t* sums,
+                                                              uint32_t nb)
+{
+	uint32_t* data = base + (size_t) blockIdx.y * stride;
+	__shared__ uint32_t sh[SCAN_THREADS];
+	const uint32_t t = threadIdx.x;
+	const size_t b0 = (size_t) blockIdx.x * SCAN_CHUNK + (size_t) t * SCAN_ITEMS;
+	uint32_t v[SCAN_ITEMS];
+	uint32_t sum = 0;
+#pragma unroll
+	for (int j = 0; j < SCAN_ITEMS; j++)
+	{
+		const size_t i = b0 + j;
+		v[j] = (i < n) ? data[i] : 0u;
+		sum += v[j];
+	}
+	sh[t] = sum;
+	__syncthreads();
+	for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1)
+	{
+		const uint32_t x = (t >= off) ? sh[t - off] : 0u;
+		__syncthreads();
+		sh[t] += x;
+		__syncthreads();
+	}
+	uint32_t run = sh[t] - sum;
+#pragma unroll
+	for (int j = 0; j < SCAN_ITEMS; j++)
+	{
+		const size_t i = b0 + j;
+		if (i < n) data[i] = run;
+		run += v[j];
+	}
+	if (t == SCAN_THREADS - 1)
+		sums[(size_t) blockIdx.y * nb + blockIdx.x] = sh[t];
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_finish(uint32_t* base, size_t stride, size_t n,
+                                                              const uint32_t* sums, uint32_t nb, uint32_t* totals)
+{
+	uint32_t* data = base + (size_t) blockIdx.y * stride;
+	const uint32_t* s = sums + (size_t) blockIdx.y * nb;
+	__shared__ uint32_t red[SCAN_THREADS];
+	const uint32_t t = threadIdx.x;
+	uint32_t acc = 0;
+	for (uint32_t k = t; k < blockIdx.x; k += SCAN_THREADS) acc += s[k];
+	red[t] = acc;
+	__syncthreads();
+	for (uint32_t off = SCAN_THREADS / 2; off > 0; off >>= 1)
+	{
+		if (t < off) red[t] += red[t + off];
+		__syncthreads();
+	}
+	const uint32_t add = red[0];
+	const size_t b0 = (size_t) blockIdx.x * SCAN_CHUNK;
+	if (add)
+		for (int j = 0; j < SCAN_ITEMS; j++)
+		{
+			const size_t i = b0 + (size_t) j * SCAN_THREADS + t;
+			if (i < n) data[i] += add;
+		}
+	if (blockIdx.x == nb - 1 && t == 0)
+		totals[blockIdx.y] = add + s[blockIdx.x];
+}
+
+void scanExclusiveMulti(uint32_t* base, size_t stride, int nArrays, size_t n, uint32_t* totals, ScanScratch& scratch,
+                        hipStream_t s)
+{
+	if (n == 0)
+	{
+		fillU32(totals, 0u, (size_t) nArrays, s);
+		return;
+	}
+	const size_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+	if (nb > 8192)
+	{
+		for (int a = 0; a < nArrays; a++)
+			scanExclusive(base + (size_t) a * stride, n, totals + a, scratch, s);
+		return;
+	}
+	const size_t need = nb * (size_t) nArrays + 2;
+	if (need > scratch.cap)
+	{
+		if (scratch.buf) hipFree(scratch.buf);
+		scratch.cap = need * 2;
+		hipMalloc(&scratch.buf, scratch.cap * sizeof(uint32_t));
+	}
+	const dim3 grid((uint32_t) nb, (uint32_t) nArrays);
+	hipLaunchKernelGGL(k_scan_blocks, grid, dim3(SCAN_THREADS), 0, s, base, stride, n, scratch.buf, (uint32_t) nb);
+	hipLaunchKernelGGL(k_scan_finish, grid, dim3(SCAN_THREADS), 0, s, base, stride, n, scratch.buf, (uint32_t) nb, totals);
+}
+
+/* Tile ranges of a list sorted by tile: [start, end) per tile; tiles that
+ * received nothing keep start == end == 0 (set by the caller's fill). */
+__global__ void k_tile_bounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint32_t k = keys[i];
+	if (i == 0 || keys[i - 1] != k) start[k] = (uint32_t) i;
+	if (i + 1 == n || keys[i + 1] != k) end[k] = (uint32_t) (i + 1);
+}
+
+void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s)
+{
+	if (n) hipLaunchKernelGGL(k_tile_bounds, dim3((uint32_t) ((n + 255) / 256)), dim3(256), 0, s, keys, n, start, end);
+}
+
 /* ------------------------------------------------------------ radix sort */
 
 constexpr int RS_THREADS = 256;
@@ -204,7 +317,7 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 	for (int shift = 0; shift < bits; shift += 8)
 	{
 		hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, (uint32_t) shift, sc.hist, nBlocks);
-		scanExclusive(sc.hist, histN, sc.hist + histN, sc.scan, s);
+		scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
 		hipLaunchKernelGGL(k_radix_scatter, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
 		                   (uint32_t) shift, sc.hist, nBlocks);
 		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
@@ -253,6 +366,34 @@ void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
 void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s)
 {
 	if (n) hipLaunchKernelGGL(k_fill_u8, fillGrid(n), dim3(256), 0, s, p, v, n);
+}
+
+/* Colour, depth and stencil fills of one framebuffer in one launch (a null
+ * array is left alone). */
+__global__ void k_fill_fb(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv, uint8_t* stencil, uint8_t sv, size_t n)
+{
+	const size_t i0 = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for (size_t i = i0; i < n; i += stride)
+	{
+		if (color) color[i] = cv;
+		if (depth) depth[i] = dv;
+	}
+	/* stencil bytes four at a time, then the tail */
+	const size_t n4 = n / 4;
+	const uint32_t sv4 = (uint32_t) sv * 0x01010101u;
+	if (stencil)
+	{
+		uint32_t* s4 = reinterpret_cast<uint32_t*>(stencil);
+		for (size_t i = i0; i < n4; i += stride) s4[i] = sv4;
+		for (size_t i = n4 * 4 + i0; i < n; i += stride) stencil[i] = sv;
+	}
+}
+
+void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv, uint8_t* stencil, uint8_t sv, size_t n,
+                     hipStream_t s)
+{
+	if (n) hipLaunchKernelGGL(k_fill_fb, fillGrid(n), dim3(256), 0, s, color, cv, depth, dv, stencil, sv, n);
 }
 
 void iota(uint32_t* p, size_t n, hipStream_t s)
