@@ -32,7 +32,6 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 from scene_runner import GPU_SO, ORACLE_SO, write_noise_png  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-RASTER_RECORD_BYTES = 96   # one SrpTri record (srp/device/abi.h)
 
 
 def load(path):
@@ -56,6 +55,15 @@ def setup(lib, args, tex):
         rc = lib.srpBenchMixSetup(args.width, args.height, args.tris, args.lines, args.points, 0x5EED0003)
     if rc != 0:
         raise RuntimeError(f"bench setup failed: {rc}")
+
+
+def geometry_bytes(args):
+    """Vertex + index bytes one frame reads (each unique vertex once)."""
+    if args.workload == "mesh":
+        nv = (args.rings + 1) * (args.segs + 1)
+        return nv * 20 + args.rings * args.segs * 6 * 4
+    # primitive mix: non-indexed {clip vec4, st vec2, level, tag} = 32 B per vertex
+    return (3 * args.tris + 2 * args.lines + args.points) * 32
 
 
 def read_frame(lib, n):
@@ -101,6 +109,21 @@ def cpu_baseline(args, tex, budget_s):
     }, frame, st[1] // frames
 
 
+def copy_probe(mib, reps=5):
+    """Plain device copy of a known size: its FETCH_SIZE / WRITE_SIZE per
+    dispatch calibrate the counters, its rate is the achievable copy BW."""
+    lib = ctypes.CDLL(GPU_SO, mode=ctypes.RTLD_LOCAL)
+    lib.srpBenchCopyProbe.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    nbytes = mib << 20
+    ms = ctypes.c_double(0.0)
+    rc = lib.srpBenchCopyProbe(nbytes, reps, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"copy probe failed: {rc}")
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "dispatches": reps + 1,
+                      "ms_per_copy": round(ms.value, 4),
+                      "GB_per_s": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,12 +140,18 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="raster tile side (8 or 16); 0 = library default")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (calibrates byte counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb)
+        return
 
     tmp = tempfile.mkdtemp(prefix="srp_bench_")
     tex = write_noise_png(os.path.join(tmp, "noise1024.png"), 1024, 0x5EED0002)
@@ -162,13 +191,16 @@ def main():
     ms_per_step = dt / steps * 1e3
     in_tris = (args.rings * args.segs * 2) if args.workload == "mesh" else args.tris
 
-    # Dominant kernel: the triangle rasteriser.  Algorithmic bytes per launch:
-    # colour+depth+stencil read and written once (W*H*9*2), one RGBA8 texel per
-    # shaded fragment (mesh workload), one setup record per assembled triangle.
+    # Dominant kernel: the triangle rasteriser.  Algorithmic bytes per launch
+    # (the per-frame figure of the workload definition): colour + depth +
+    # stencil of the frame read and written once (W*H*9*2) plus one RGBA8
+    # texel per shaded fragment (mesh workload).  The whole frame adds the
+    # vertex and index data: B_frame = VB + IB + W*H*9*2 + shaded*T_tex.
     raster_ms = ms[0] / max(launches[0], 1)
     tex_bytes = 4 if args.workload == "mesh" else 0
-    raster_bytes = n_px * 9 * 2 + shaded_per_frame * tex_bytes + prims_per_frame * RASTER_RECORD_BYTES
+    raster_bytes = n_px * 9 * 2 + shaded_per_frame * tex_bytes
     achieved = raster_bytes / (raster_ms * 1e-3) / 1e9 if raster_ms > 0 else 0.0
+    frame_bytes = raster_bytes + geometry_bytes(args)
     traffic = None
     tpath = os.path.join(REPO, "profiles", f"raster_{args.workload}_traffic.json")
     if os.path.exists(tpath):
@@ -208,13 +240,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "srp_raster_kernel",
+            "kernel": "srp_raster_batched",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
             "algorithmic_bytes": int(raster_bytes),
+            "frame_bytes": int(frame_bytes),
+            "frame_achieved": round(frame_bytes / (ms_per_step * 1e-3) / 1e9, 2),
         },
         "cpu_baseline": None,
     }

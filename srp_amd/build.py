@@ -104,8 +104,8 @@ def build_scenes(jobs=8):
     """The GPU build of the test scene programs (tests/scenes)."""
     host = os.path.join(CSRC, "host")
     return build_shader_module(
-        [os.path.join(SCENE_DIR, "device_shaders.hip")],
-        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c", "bench.c", "messages.c")] + [os.path.join(host, "obj_load.c")],
+        [os.path.join(SCENE_DIR, "device_shaders.hip"), os.path.join(SCENE_DIR, "copy_probe.hip")],
+        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c", "bench.c", "messages.c", "terrain.c")] + [os.path.join(host, "obj_load.c")],
         SCENE_LIB, includes=[SCENE_DIR, host], jobs=jobs, defines=["SRP_GPU_BUILD"])
 
 
