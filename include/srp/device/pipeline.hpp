@@ -451,6 +451,35 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
  *      fragment as shaded only where the reference invokes the shader. */
 constexpr int RF = 256;
 
+/* Fragment shaders that evaluate a whole wave of fragments together (all 64
+ * lanes call FS().wave(in, out, active) convergently) declare
+ * `static constexpr bool kWave = true`; the batched rasteriser then calls
+ * that entry, the other kernels the per-fragment one. */
+template <class FS, class = void>
+struct FsIsWave
+{
+	static constexpr bool value = false;
+};
+template <class FS>
+struct FsIsWave<FS, decltype((void) FS::kWave)>
+{
+	static constexpr bool value = FS::kWave;
+};
+
+/* Interpolated-varying buffer of a fragment: words or bytes. */
+template <bool WORDS>
+struct VaryingBuf
+{
+	typedef uint32_t type;
+	static constexpr uint32_t n = SRP_MAX_WORDS;
+};
+template <>
+struct VaryingBuf<false>
+{
+	typedef uint32_t type;
+	static constexpr uint32_t n = SRP_MAX_VARYING_BYTES / 4;
+};
+
 enum : uint8_t { FRAG_SCISSORED = 1, FRAG_CULLED = 2, FRAG_SHADED = 3 };
 
 __device__ __forceinline__ bool monotoneOp(uint32_t op)
@@ -613,6 +642,15 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 			const uint32_t f = f0 + lane;
 			const uint32_t p = (f < nFrag) ? fragPix[f] : 0u;
 			const float pixDepth = __shfl(dep, (int) p);
+			uint8_t state = FRAG_CULLED;
+			float depth = 0.0f;
+			SRPFragmentShaderIn in;
+			SRPFragmentShaderOut out;
+			out.color[0] = 0.0f; out.color[1] = 0.0f; out.color[2] = 0.0f; out.color[3] = 0.0f;
+			out.fragDepth = __builtin_nanf("");
+			typedef typename VaryingBuf<WORDS>::type BufT;
+			BufT buf[VaryingBuf<WORDS>::n];
+			in.varyings = (SRPInterpolated*) buf;
 			if (f < nFrag)
 			{
 				const uint32_t b = fragTri[f];
@@ -629,8 +667,8 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 				}
 				const float iw[3] = {T.invW[0], T.invW[1], T.invW[2]};
 				const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
-				float depth = T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2];
-				uint8_t state = FRAG_SHADED;
+				depth = T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2];
+				state = FRAG_SHADED;
 				if (st.scissorEnabled)
 				{
 					const uint64_t ux = (uint64_t) (int64_t) fx, uy = (uint64_t) (int64_t) fy;
@@ -641,7 +679,6 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 					state = FRAG_CULLED;
 				if (state == FRAG_SHADED)
 				{
-					SRPFragmentShaderIn in;
 					in.uniform = (SRPUniform*) P->uniform;
 					in.fragCoord[0] = (float) ((double) fx + 0.5);
 					in.fragCoord[1] = (float) ((double) fy + 0.5);
@@ -649,30 +686,44 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 					in.fragCoord[3] = rec;
 					in.frontFacing = (T.flags >> 3) & 1u;
 					in.primitiveID = T.id;
-					SRPFragmentShaderOut out;
-					out.color[0] = 0.0f; out.color[1] = 0.0f; out.color[2] = 0.0f; out.color[3] = 0.0f;
-					out.fragDepth = __builtin_nanf("");
 					if (WORDS)
 					{
-						uint32_t buf[SRP_MAX_WORDS];
 						const uint32_t* v[3];
 #pragma unroll
 						for (int k = 0; k < 3; k++) v[k] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T.vref[k]));
-						interpWords<3>(P, v, l, iw, rec, pv, buf);
-						in.varyings = (SRPInterpolated*) buf;
-						FS()(&in, &out);
+						interpWords<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint32_t*>(buf));
 					}
 					else
 					{
-						uint8_t buf[SRP_MAX_VARYING_BYTES];
 						const uint8_t* v[3];
 #pragma unroll
 						for (int k = 0; k < 3; k++) v[k] = varyingPtr(P, T.vref[k]);
-						for (uint32_t bb = 0; bb < P->describedSize; bb++) buf[bb] = 0;
-						interpBytes<3>(P, v, l, iw, rec, pv, buf);
-						in.varyings = (SRPInterpolated*) buf;
-						FS()(&in, &out);
+						uint8_t* bb = reinterpret_cast<uint8_t*>(buf);
+						for (uint32_t q = 0; q < P->describedSize; q++) bb[q] = 0;
+						interpBytes<3>(P, v, l, iw, rec, pv, bb);
 					}
+				}
+			}
+			const bool active = state == FRAG_SHADED;
+			if constexpr (FsIsWave<FS>::value)
+			{
+				/* every lane calls; lanes without a fragment pass zeroed inputs */
+				if (!active)
+				{
+					in.uniform = (SRPUniform*) P->uniform;
+					in.fragCoord[0] = 0.0f; in.fragCoord[1] = 0.0f; in.fragCoord[2] = 0.0f; in.fragCoord[3] = 0.0f;
+					in.frontFacing = 0;
+					in.primitiveID = 0;
+					for (uint32_t q = 0; q < VaryingBuf<WORDS>::n; q++) buf[q] = 0;
+				}
+				FS().wave(&in, &out, active);
+			}
+			else if (active)
+				FS()(&in, &out);
+			if (f < nFrag)
+			{
+				if (active)
+				{
 					if (LATE && !isnan(out.fragDepth))
 						depth = out.fragDepth;
 					fragCol[f] = packColor(out.color);

@@ -15,7 +15,17 @@
  * `uniformSize` is sizeof the uniform struct (its bytes are copied to the
  * device at every draw; an SRPTexture* inside it is already a device
  * handle).  FP contraction is switched off for the rest of the file, as
- * bit-exact results require. */
+ * bit-exact results require.
+ *
+ *   SRP_HIP_FRAGMENT_SHADER_WAVE(symbol, fn, waveFn, uniformSize)
+ *
+ * registers a fragment shader that can also shade a whole wave of 64
+ * fragments together (matrix-core work such as the neural shader of
+ * srp/device/neural.hpp): waveFn(in, out, active) is called by all 64
+ * lanes of the batched triangle rasteriser at once, `active` false on
+ * lanes without a fragment (their inputs are zeroed and their output is
+ * discarded); fn serves the kernels whose lanes are not convergent (lines,
+ * points, the unbatched rasteriser). */
 #pragma once
 
 #include "srp/srp.h"
@@ -44,14 +54,7 @@ extern "C" void srpHipShaderCalledOnHost(const char* name);
 	static const SrpVsEntry srpVsEntry_##symbol = {SRP_ABI_VERSION, (size_t) (uniformSize), srpVsLaunch_##symbol}; \
 	static const int srpVsReg_##symbol = (srpHipRegisterVertexShader((const void*) &symbol, &srpVsEntry_##symbol), 0);
 
-#define SRP_HIP_FRAGMENT_SHADER(symbol, fn, uniformSize)                                          \
-	struct SrpFsFunctor_##symbol                                                                  \
-	{                                                                                             \
-		__device__ __forceinline__ void operator()(SRPFragmentShaderIn* i, SRPFragmentShaderOut* o) const \
-		{                                                                                         \
-			fn(i, o);                                                                             \
-		}                                                                                         \
-	};                                                                                            \
+#define SRP_HIP_FS_REGISTER_(symbol, uniformSize)                                                 \
 	extern "C" void symbol(SRPFragmentShaderIn*, SRPFragmentShaderOut*) { srpHipShaderCalledOnHost(#symbol); } \
 	static void srpFsRaster_##symbol(const SrpRasterLaunch* l, void* s)                           \
 	{                                                                                             \
@@ -64,3 +67,28 @@ extern "C" void srpHipShaderCalledOnHost(const char* name);
 	static const SrpFsEntry srpFsEntry_##symbol = {SRP_ABI_VERSION, (size_t) (uniformSize),       \
 	                                               srpFsRaster_##symbol, srpFsFrags_##symbol};    \
 	static const int srpFsReg_##symbol = (srpHipRegisterFragmentShader((const void*) &symbol, &srpFsEntry_##symbol), 0);
+
+#define SRP_HIP_FRAGMENT_SHADER_WAVE(symbol, fn, waveFn, uniformSize)                             \
+	struct SrpFsFunctor_##symbol                                                                  \
+	{                                                                                             \
+		static constexpr bool kWave = true;                                                       \
+		__device__ __forceinline__ void operator()(SRPFragmentShaderIn* i, SRPFragmentShaderOut* o) const \
+		{                                                                                         \
+			fn(i, o);                                                                             \
+		}                                                                                         \
+		__device__ __forceinline__ void wave(SRPFragmentShaderIn* i, SRPFragmentShaderOut* o, bool active) const \
+		{                                                                                         \
+			waveFn(i, o, active);                                                                 \
+		}                                                                                         \
+	};                                                                                            \
+	SRP_HIP_FS_REGISTER_(symbol, uniformSize)
+
+#define SRP_HIP_FRAGMENT_SHADER(symbol, fn, uniformSize)                                          \
+	struct SrpFsFunctor_##symbol                                                                  \
+	{                                                                                             \
+		__device__ __forceinline__ void operator()(SRPFragmentShaderIn* i, SRPFragmentShaderOut* o) const \
+		{                                                                                         \
+			fn(i, o);                                                                             \
+		}                                                                                         \
+	};                                                                                            \
+	SRP_HIP_FS_REGISTER_(symbol, uniformSize)

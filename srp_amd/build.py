@@ -60,7 +60,7 @@ def build_library(jobs=8):
     """libsrp_amd.so: host API + shader-independent kernels."""
     os.makedirs(OBJ, exist_ok=True)
     inc = [INCLUDE, os.path.join(CSRC, "lib"), os.path.join(CSRC, "host")]
-    hip_srcs = ["lib/runtime.cpp", "lib/assemble.hip", "lib/util.hip"]
+    hip_srcs = ["lib/runtime.cpp", "lib/assemble.hip", "lib/util.hip", "neural/mlp.hip"]
     c_srcs = ["host/math.c", "host/png_decode.c", "host/obj_load.c"]
     tasks = []
     with ThreadPoolExecutor(max_workers=jobs) as ex:

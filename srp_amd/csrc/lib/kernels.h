@@ -5,6 +5,8 @@
 #include <stdint.h>
 
 #include "srp/buffer.h"
+#include "srp/framebuffer.h"
+#include "srp/message_callback.h"
 #include "srp/device/abi.h"
 #include "srp/device/pipeline.hpp"
 
@@ -79,6 +81,14 @@ void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void iota(uint32_t* p, size_t n, hipStream_t s);
 void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv, uint8_t* stencil, uint8_t sv, size_t n,
                      hipStream_t s);
+
+/* runtime.cpp: the library stream, and the device arrays of a framebuffer
+ * (pending deferred fills applied first; a synchronous framebuffer's host
+ * arrays are uploaded first). */
+hipStream_t libStream();
+/* Report through the user's message callback (printf-style). */
+void libMessage(SRPMessageType type, SRPMessageSeverity sev, const char* fn, const char* fmt, ...);
+void framebufferDevice(const SRPFramebuffer* fb, uint32_t** color, float** depth, uint8_t** stencil);
 
 }  // namespace lib
 }  // namespace srp

@@ -290,6 +290,36 @@ void texPush(TexImpl* T)
 
 }  // namespace
 
+namespace srp {
+namespace lib {
+
+hipStream_t libStream() { return rt().stream; }
+
+void libMessage(SRPMessageType type, SRPMessageSeverity sev, const char* fn, const char* fmt, ...)
+{
+	char text[1024];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(text, sizeof(text), fmt, ap);
+	va_end(ap);
+	message(type, sev, fn, "%s", text);
+}
+
+void framebufferDevice(const SRPFramebuffer* fb, uint32_t** color, float** depth, uint8_t** stencil)
+{
+	FbImpl* F = fbImpl(fb);
+	if (F->deferred)
+		fbFlushFills(F);
+	else
+		fbUpload(F);
+	if (color) *color = F->dColor;
+	if (depth) *depth = F->dDepth;
+	if (stencil) *stencil = F->dStencil;
+}
+
+}  // namespace lib
+}  // namespace srp
+
 struct SRPVertexBuffer
 {
 	size_t nBytesPerVertex = 0, nVertices = 0, nBytesAllocated = 0;

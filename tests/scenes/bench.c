@@ -9,8 +9,10 @@
 
 #include "shaders.h"
 #include "scenes.h"
+#include "terrain.h"
 #ifdef SRP_GPU_BUILD
 #include <srp/srp_hip.h>
+#include <srp/srp_neural.h>
 #endif
 
 void sc_vsMeshTex(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
@@ -18,6 +20,9 @@ void sc_fsMeshTex(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
 void sc_vsMix(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
 void sc_fsMix(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
 void sc_vsColorPass(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
+void sc_vsTerrain(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
+void sc_fsTerrain(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
+void sc_fsTerrainCapture(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
 void sc_fsColor(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
 
 void srpSceneBuildSphere(int rings, int segs, VtxPosUV** verts, size_t* nVerts, uint32_t** idx, size_t* nIdx);
@@ -32,12 +37,14 @@ extern SRPVaryingInfo g_mixVaryings[3];
 
 typedef struct Bench
 {
-	int kind;  /* 0 mesh, 1 mix */
-	SRPFramebuffer* fb;
+	int kind;  /* 0 mesh, 1 mix, 2 terrain */
+	SRPFramebuffer* fb;       /* current target: fbs[0] or fbs[1] */
+	SRPFramebuffer* fbs[2];
 	SRPVertexBuffer *vb, *vbL, *vbP;
 	SRPIndexBuffer* ib;
 	size_t nIdx, nTri, nLine, nPoint;
 	UniFrameMVPTex u;
+	UniTerrain ut;
 	SRPVertexShader vs;
 	SRPFragmentShader fs;
 	SRPShaderProgram sp;
@@ -51,6 +58,7 @@ static void benchFbInit(int w, int h)
 	g_b.width = w;
 	g_b.height = h;
 	g_b.fb = srpNewFramebuffer((size_t) w, (size_t) h);
+	g_b.fbs[0] = g_b.fb;
 	memset(g_b.fb->stencil, 0, g_b.fb->size);
 #ifdef SRP_GPU_BUILD
 	srpFramebufferSetDeferred(g_b.fb, true);
@@ -104,6 +112,111 @@ int srpBenchMixSetup(int w, int h, int nTri, int nLine, int nPoint, uint64_t see
 	return 0;
 }
 
+/* Terrain heightfield of nx x nz cells seen from `pose`, with the
+ * procedural material (capture = 0) or G-buffer capture pass capture-1. */
+int srpBenchTerrainSetup(int w, int h, int nx, int nz, int pose, int capture)
+{
+	memset(&g_b, 0, sizeof(g_b));
+	g_b.kind = 2;
+	if (capture < 0 || capture > 4) return -1;
+	srpNewContext(&srpContext);
+	VtxTerrain* verts; uint32_t* idx; size_t nv, ni;
+	srpTerrainBuild(nx, nz, &verts, &nv, &idx, &ni, &g_b.ut.bounds);
+	g_b.vb = srpNewVertexBuffer();
+	g_b.ib = srpNewIndexBuffer();
+	srpVertexBufferCopyData(g_b.vb, sizeof(VtxTerrain), nv * sizeof(VtxTerrain), verts);
+	srpIndexBufferCopyData(g_b.ib, SRP_UINT32, ni * sizeof(uint32_t), idx);
+	free(verts); free(idx);
+	g_b.nIdx = ni;
+	g_b.width = w; g_b.height = h;
+	const TerrainBounds b = g_b.ut.bounds;
+	srpTerrainPose(pose, w, h, &g_b.ut);
+	g_b.ut.bounds = b;
+	g_b.ut.pass = capture ? (uint32_t) capture - 1u : 0u;
+	g_b.vs.shader = sc_vsTerrain; g_b.vs.nVaryings = 3; g_b.vs.varyingsInfo = g_terrainVaryings; g_b.vs.varyingsSize = sizeof(VarTerrain);
+	g_b.fs.shader = capture ? sc_fsTerrainCapture : sc_fsTerrain; g_b.fs.mayOverwriteDepth = false;
+	g_b.sp.uniform = (SRPUniform*) &g_b.ut; g_b.sp.vs = &g_b.vs; g_b.sp.fs = &g_b.fs;
+	benchFbInit(w, h);
+	srpTerrainState();
+	return 0;
+}
+
+/* Switch the terrain workload's camera pose and fragment shader:
+ * fs = 0 procedural, 1+k capture pass k, or an explicit shader function
+ * with its uniform network pointer (neural shading). */
+void srpBenchTerrainPose(int pose)
+{
+	const TerrainBounds b = g_b.ut.bounds;
+	const uint32_t pass = g_b.ut.pass;
+	const void* net = g_b.ut.network;
+	srpTerrainPose(pose, g_b.width, g_b.height, &g_b.ut);
+	g_b.ut.bounds = b;
+	g_b.ut.pass = pass;
+	g_b.ut.network = net;
+}
+
+void srpBenchTerrainShader(int capture, void (*fs)(SRPFragmentShaderIn*, SRPFragmentShaderOut*), const void* network)
+{
+	if (fs)
+		g_b.fs.shader = fs;
+	else
+		g_b.fs.shader = capture ? sc_fsTerrainCapture : sc_fsTerrain;
+	g_b.ut.pass = capture ? (uint32_t) capture - 1u : 0u;
+	g_b.ut.network = network;
+}
+
+const SRPFramebuffer* srpBenchFramebuffer(void) { return g_b.fb; }
+
+/* Select the framebuffer later frames render into: 0 the workload's, 1 a
+ * second one of the same size (created on first use). */
+void srpBenchUseFramebuffer(int which)
+{
+	if (which == 1 && !g_b.fbs[1])
+	{
+		g_b.fbs[1] = srpNewFramebuffer((size_t) g_b.width, (size_t) g_b.height);
+		memset(g_b.fbs[1]->stencil, 0, g_b.fbs[1]->size);
+#ifdef SRP_GPU_BUILD
+		srpFramebufferSetDeferred(g_b.fbs[1], true);
+#endif
+	}
+	g_b.fb = g_b.fbs[which ? 1 : 0];
+}
+
+#ifdef SRP_GPU_BUILD
+/* Training data of the neural material: for each pose, the four G-buffer
+ * capture passes and the procedural shading through the ordinary pipeline,
+ * then the covered pixels kept with probability `keep`.  Returns the
+ * samples appended.  Leaves the procedural shader selected. */
+size_t srpBenchTerrainCollect(SRPNeuralDataset* ds, int firstPose, int nPoses, float keep, uint64_t seed)
+{
+	size_t total = 0;
+	for (int p = firstPose; p < firstPose + nPoses; p++)
+	{
+		srpBenchTerrainPose(p);
+		for (int k = 0; k < 4; k++)
+		{
+			srpBenchTerrainShader(k + 1, NULL, NULL);
+			srpBenchFrame();
+			srpNeuralDatasetStage(ds, g_b.fb, k);
+		}
+		srpBenchTerrainShader(0, NULL, NULL);
+		srpBenchFrame();
+		srpNeuralDatasetStage(ds, g_b.fb, 4);
+		total += srpNeuralDatasetAppend(ds, g_b.fb, keep, seed + (uint64_t) p);
+	}
+	return total;
+}
+#endif
+
+void srpBenchTerrainBounds(float out[6])
+{
+	for (int i = 0; i < 3; i++)
+	{
+		out[i] = g_b.ut.bounds.lo[i];
+		out[3 + i] = g_b.ut.bounds.inv[i];
+	}
+}
+
 void srpBenchFrame(void)
 {
 	srpFramebufferClear(g_b.fb);
@@ -114,7 +227,7 @@ void srpBenchFrame(void)
 	for (size_t i = 0; i < g_b.fb->size; i++) g_b.fb->depth[i] = 1.0f;
 	memset(g_b.fb->stencil, 0, g_b.fb->size);
 #endif
-	if (g_b.kind == 0)
+	if (g_b.kind == 0 || g_b.kind == 2)
 		srpDrawIndexBuffer(g_b.ib, g_b.vb, g_b.fb, &g_b.sp, SRP_PRIM_TRIANGLES, 0, g_b.nIdx);
 	else
 		srpSceneMixDraw(g_b.fb, &g_b.sp, g_b.vb, g_b.vbL, g_b.vbP, g_b.nTri, g_b.nLine, g_b.nPoint);
@@ -138,6 +251,7 @@ void srpBenchFinish(void)
 	if (g_b.vb) srpFreeVertexBuffer(g_b.vb);
 	if (g_b.vbL) srpFreeVertexBuffer(g_b.vbL);
 	if (g_b.vbP) srpFreeVertexBuffer(g_b.vbP);
-	if (g_b.fb) srpFreeFramebuffer(g_b.fb);
+	if (g_b.fbs[0]) srpFreeFramebuffer(g_b.fbs[0]);
+	if (g_b.fbs[1]) srpFreeFramebuffer(g_b.fbs[1]);
 	memset(&g_b, 0, sizeof(g_b));
 }

@@ -16,6 +16,7 @@
 #include "shaders.h"
 #include "obj_load.h"
 #include "scenes.h"
+#include "terrain.h"
 
 SRPContext srpContext;
 
@@ -658,6 +659,39 @@ static int scCfgMix(const SceneArgs* a, SceneOut* o)
 	return scFinish(fb, o);
 }
 
+/* Terrain heightfield of p0 x p1 cells (default 1000 x 500) from pose p2;
+ * seed selects the fragment shader: 0 the procedural material, 1+k the
+ * G-buffer capture pass k. */
+static int scCfgTerrain(const SceneArgs* a, SceneOut* o)
+{
+	const int w = a->width ? a->width : 1920, h = a->height ? a->height : 1080;
+	VtxTerrain* verts; uint32_t* idx; size_t nv, ni;
+	UniTerrain u;
+	memset(&u, 0, sizeof(u));
+	srpTerrainBuild(a->p0 > 0 ? a->p0 : 1000, a->p1 > 0 ? a->p1 : 500, &verts, &nv, &idx, &ni, &u.bounds);
+	const TerrainBounds b = u.bounds;
+	srpTerrainPose(a->p2 > 0 ? a->p2 : 0, w, h, &u);
+	u.bounds = b;
+	if (a->seed > 4) { free(verts); free(idx); return -1; }
+	u.pass = a->seed ? (uint32_t) a->seed - 1u : 0u;
+	SRPVertexShader vs = scVS(sc_vsTerrain, 3, g_terrainVaryings, sizeof(VarTerrain));
+	SRPFragmentShader fs = scFS(a->seed ? sc_fsTerrainCapture : sc_fsTerrain, false);
+	SRPShaderProgram sp = scProgram(&u, &vs, &fs);
+	SRPFramebuffer* fb = scBegin(a, (size_t) w, (size_t) h);
+	srpTerrainState();
+	SRPVertexBuffer* vb = srpNewVertexBuffer();
+	SRPIndexBuffer* ib = srpNewIndexBuffer();
+	srpVertexBufferCopyData(vb, sizeof(VtxTerrain), nv * sizeof(VtxTerrain), verts);
+	srpIndexBufferCopyData(ib, SRP_UINT32, ni * sizeof(uint32_t), idx);
+	srpFramebufferClear(fb);
+	scDepthInit(fb, 1.0f);
+	srpDrawIndexBuffer(ib, vb, fb, &sp, SRP_PRIM_TRIANGLES, 0, ni);
+	srpFreeIndexBuffer(ib);
+	srpFreeVertexBuffer(vb);
+	free(verts); free(idx);
+	return scFinish(fb, o);
+}
+
 /* ----------------------------------------------------------- registry */
 
 typedef struct SceneEntry { const char* name; int (*fn)(const SceneArgs*, SceneOut*); } SceneEntry;
@@ -685,6 +719,7 @@ static const SceneEntry kScenes[] = {
 	{"cfg/mesh", scCfgMesh},
 	{"cfg/mix", scCfgMix},
 	{"cfg/fuzz", srpSceneFuzz},
+	{"cfg/terrain", scCfgTerrain},
 };
 
 int srpSceneCount(void) { return (int) (sizeof(kScenes) / sizeof(kScenes[0])); }

@@ -269,3 +269,5 @@ SCENE_SHADER void fsFuzzDepth(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out
 	scFuzzColor(in, out);
 	out->fragDepth = (i->f[2] > 0.5f) ? NAN : i->f[2] * 2.0f - 1.0f;
 }
+
+#include "terrain_shaders.h"

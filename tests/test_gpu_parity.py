@@ -76,6 +76,21 @@ def noise_tex(tmp_path_factory):
     return write_noise_png(str(tmp_path_factory.mktemp("tex") / "noise1024.png"), 1024, 0x5EED0002)
 
 
+@pytest.mark.parametrize("shader", [0, 1, 2, 3, 4])
+def test_cfg_terrain_small(oracle, gpu, shader):
+    """Terrain heightfield 200x100 cells, 480x270: procedural material
+    (shader 0) and the four G-buffer capture passes."""
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/terrain", width=480, height=270, p0=200, p1=100, p2=5, seed=shader)
+    assert_same(a, b, sa, sb, f"cfg/terrain shader {shader}")
+
+
+@pytest.mark.parametrize("pose", [0, 256])
+def test_cfg_terrain_full_grid(oracle, gpu, pose):
+    """The 1000x500-cell terrain (1,000,000 triangles) at 960x540."""
+    a, b, sa, sb = run_both(oracle, gpu, "cfg/terrain", width=960, height=540, p2=pose)
+    assert_same(a, b, sa, sb, f"cfg/terrain pose {pose}")
+
+
 def test_cfg_mesh_small(oracle, gpu, noise_tex):
     a, b, sa, sb = run_both(oracle, gpu, "cfg/mesh", width=480, height=270, p0=32, p1=64, tex_path=noise_tex)
     assert_same(a, b, sa, sb, "cfg/mesh small")
