@@ -109,6 +109,117 @@ def cpu_baseline(args, tex, budget_s):
     }, frame, st[1] // frames
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# algorithmic FLOPs of the network per fragment: 2*(512*64 + 64*64 + 64*3)
+NN_FWD_FLOPS = 74112
+# forward + input gradients of layers 3, 2 + weight gradients of all layers
+NN_TRAIN_FLOPS = NN_FWD_FLOPS + 2 * (64 * 3 + 64 * 64 + 512 * 64 + 64 * 64 + 3 * 64)
+
+
+def neural_leg(args):
+    """The terrain's neural fragment shader: capture training fragments from
+    256 poses with the ordinary pipeline, train the network on the GPU,
+    evaluate it against the procedural shader on 64 held-out poses, and time
+    frames with each shader."""
+    from neural_runner import Neural
+    nn = Neural()
+    W, H = args.width, args.height
+    if nn.terrain_setup(W, H, args.grid_x, args.grid_z, 0, 0) != 0:
+        raise RuntimeError("terrain setup failed")
+    ds = nn.new_ds(8_000_000)
+    net = nn.new_net(0x5EED0005)
+    out = {}
+    try:
+        t0 = time.perf_counter()
+        got = nn.terrain_collect(ds, 0, 256, args.nn_keep, 0x5EED0005)
+        nn.sync()
+        out["train_fragments"] = int(got)
+        out["capture_s"] = round(time.perf_counter() - t0, 3)
+        dev = nn.net_device(net)
+
+        def evaluate(poses):
+            ps, mx = [], []
+            for pose in poses:
+                nn.terrain_pose(pose)
+                nn.use_fb(0)
+                nn.terrain_shader(0, None, None)
+                nn.frame()
+                fa = nn.fb()
+                nn.use_fb(1)
+                nn.terrain_shader(0, nn.neural_fs, dev)
+                nn.frame()
+                p, m, _ = nn.compare(fa, nn.fb(), fa)
+                ps.append(p)
+                mx.append(m)
+            nn.use_fb(0)
+            return ps, mx
+
+        test_poses = range(256, 256 + 64)
+        ps0, _ = evaluate(test_poses[:8])
+        nn.sync()
+        t0 = time.perf_counter()
+        loss = nn.train(net, ds, args.nn_steps, args.nn_batch, args.nn_lr, args.nn_lr * 0.02, 0x5EED0005)
+        t_train = time.perf_counter() - t0
+        samples = args.nn_steps * args.nn_batch
+        out["train"] = {"steps": args.nn_steps, "batch": args.nn_batch, "s": round(t_train, 3),
+                        "ms_per_step": round(t_train / args.nn_steps * 1e3, 4),
+                        "loss_first": float(loss[0]), "loss_last": float(np.mean(loss[-50:])),
+                        "mfma_tflops": round(samples * NN_TRAIN_FLOPS / t_train / 1e12, 2),
+                        "mfma_frac": round(samples * NN_TRAIN_FLOPS / t_train / 1e12 / MFMA_BF16_PEAK_TFLOPS, 5)}
+        ps, mx = evaluate(test_poses)
+        out["psnr_db_mean"] = round(float(np.mean(ps)), 3)
+        out["psnr_db_min"] = round(float(np.min(ps)), 3)
+        out["max_abs_err"] = float(np.max(mx))
+        out["psnr_db_untrained"] = round(float(np.mean(ps0)), 3)
+        out["eval_poses"] = len(test_poses)
+
+        def time_frames(capture, fs, netdev, pose=256):
+            nn.use_fb(0)
+            nn.terrain_pose(pose)
+            nn.terrain_shader(capture, fs, netdev)
+            for _ in range(3):
+                nn.frame()
+            nn.sync()
+            nn.reset_stats()
+            nn.profile_enable(True)
+            nn.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.nn_frames):
+                nn.frame()
+            nn.sync()
+            dt = time.perf_counter() - t0
+            ms, n = nn.profile_read()
+            nn.profile_enable(False)
+            st = nn.stats()
+            return dt / args.nn_frames * 1e3, ms[0] / max(n[0], 1), st[1] // args.nn_frames
+
+        plain_ms, plain_raster, shaded = time_frames(0, None, None)
+        neural_ms, neural_raster, shaded_n = time_frames(0, nn.neural_fs, dev)
+        # the same frame with a trivial fragment shader (capture pass 0): what
+        # the rasteriser costs without shading work
+        base_ms, base_raster, _ = time_frames(1, None, None)
+        out["frame_ms_trivial_fs"] = round(base_ms, 4)
+        out["raster_ms_trivial_fs"] = round(base_raster, 4)
+        out["fs_ms_plain"] = round(plain_raster - base_raster, 4)
+        out["fs_ms_neural"] = round(neural_raster - base_raster, 4)
+        out["neural_speedup_fs"] = round((plain_raster - base_raster) / max(neural_raster - base_raster, 1e-6), 3)
+        out["frame_ms_plain"] = round(plain_ms, 4)
+        out["frame_ms_neural"] = round(neural_ms, 4)
+        out["raster_ms_plain"] = round(plain_raster, 4)
+        out["raster_ms_neural"] = round(neural_raster, 4)
+        out["shaded_per_frame"] = int(shaded)
+        out["neural_speedup_frame"] = round(plain_ms / neural_ms, 3)
+        out["neural_speedup_raster"] = round(plain_raster / neural_raster, 3)
+        out["neural_raster_mfma_tflops"] = round(shaded_n * NN_FWD_FLOPS / (neural_raster * 1e-3) / 1e12, 2)
+        out["neural_raster_mfma_frac"] = round(shaded_n * NN_FWD_FLOPS / (neural_raster * 1e-3) / 1e12
+                                               / MFMA_BF16_PEAK_TFLOPS, 5)
+    finally:
+        nn.free_net(net)
+        nn.free_ds(ds)
+        nn.finish()
+    return out
+
+
 def copy_probe(mib, reps=5):
     """Plain device copy of a known size: its FETCH_SIZE / WRITE_SIZE per
     dispatch calibrate the counters, its rate is the achievable copy BW."""
@@ -140,6 +251,15 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="raster tile side (8 or 16); 0 = library default")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--grid-x", type=int, default=1000, help="terrain cells along x (neural leg)")
+    ap.add_argument("--grid-z", type=int, default=500, help="terrain cells along z (neural leg)")
+    ap.add_argument("--nn-keep", type=float, default=0.0135, help="fraction of covered pixels kept per training pose")
+    ap.add_argument("--nn-steps", type=int, default=150000)
+    ap.add_argument("--nn-batch", type=int, default=8192)
+    ap.add_argument("--nn-lr", type=float, default=5e-3)
+    ap.add_argument("--nn-frames", type=int, default=50)
+    ap.add_argument("--neural-only", action="store_true", help="run only the neural-shader leg and print its JSON")
+    ap.add_argument("--no-neural", action="store_true", help="skip the neural-shader leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates byte counters) and exit")
     args = ap.parse_args()
@@ -151,6 +271,9 @@ def main():
 
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
+        return
+    if args.neural_only:
+        print(json.dumps({"neural": neural_leg(args)}))
         return
 
     tmp = tempfile.mkdtemp(prefix="srp_bench_")
@@ -259,6 +382,10 @@ def main():
         same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(gframe, cframe))
         result["parity_vs_oracle"] = bool(same and cpu_shaded == shaded_per_frame)
         result["speedup_vs_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
+
+    if not args.no_neural and args.workload == "mesh":
+        print("bench.py: neural shader leg (capture, training, evaluation)", file=sys.stderr, flush=True)
+        result["neural"] = neural_leg(args)
 
     print(json.dumps(result))
 
