@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 6u
+#define SRP_ABI_VERSION 7u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -124,15 +124,33 @@ typedef struct SrpDrawParams
 
 	/* tiles */
 	uint32_t tileShift, tilesX, tilesY;
+	uint32_t maskBox;        /* triangles whose box is at most this many pixels
+	                            on each side get exact 8x8 masks at binning */
 
 	unsigned long long* stats;   /* covered, shaded, primitives, draws */
 } SrpDrawParams;
+
+/* One draw of a multi-draw launch: its parameters and its range of the
+ * launch's work items (cache slots, or tiles in the global tile numbering)
+ * and of its blocks. */
+typedef struct SrpSeg
+{
+	const SrpDrawParams* P;
+	uint32_t first;        /* first work item of the draw */
+	uint32_t count;        /* work items of the draw */
+	uint32_t firstBlock;   /* first block of the draw (vertex stage) */
+	uint32_t pad;
+} SrpSeg;
 
 /* Launch arguments of the shader-module kernels. */
 typedef struct SrpVsLaunch
 {
 	const SrpDrawParams* params;
 	uint32_t count;
+	/* multi-draw: nSeg > 0 draws, blocks split by SrpSeg.firstBlock */
+	const SrpSeg* seg;
+	uint32_t nSeg;
+	uint32_t nBlocks;
 } SrpVsLaunch;
 
 typedef struct SrpRasterLaunch
@@ -150,6 +168,14 @@ typedef struct SrpRasterLaunch
 	uint32_t tile;              /* tile side: 8 or 16 */
 	uint32_t nBlocks;           /* grid size (for the XCD-aware tile order) */
 	uint32_t batched;           /* batched three-phase kernel (tile 8, one wave per tile) */
+	/* multi-draw persistent mode (nSeg > 0): nTiles counts the tiles of all
+	 * draws, seg[d] covers draw d's tiles [first, first + count), and the
+	 * nBlocks waves take tiles from `queues` (8 counters, one per range of
+	 * tiles when xcdQueues, else only queues[0]) until all are done. */
+	const SrpSeg* seg;
+	uint32_t nSeg;
+	uint32_t xcdQueues;
+	uint32_t* queues;
 } SrpRasterLaunch;
 
 typedef struct SrpFragLaunch

@@ -311,10 +311,24 @@ __device__ __forceinline__ void flushStats(const SrpDrawParams* P, uint32_t cove
 template <class VS>
 __global__ __launch_bounds__(256) void srp_vs_kernel(SrpVsLaunch L)
 {
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if (i >= L.count)
-		return;
 	const SrpDrawParams* P = L.params;
+	uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	uint32_t count = L.count;
+	if (L.nSeg)
+	{
+		uint32_t a = 0, b = L.nSeg;
+		while (b - a > 1)
+		{
+			const uint32_t m = (a + b) >> 1;
+			if (L.seg[m].firstBlock <= blockIdx.x) a = m;
+			else b = m;
+		}
+		P = L.seg[a].P;
+		i = (blockIdx.x - L.seg[a].firstBlock) * 256u + threadIdx.x;
+		count = L.seg[a].count;
+	}
+	if (i >= count)
+		return;
 	const uint64_t vi = P->cacheBase + i;
 	SRPVertexShaderIn in;
 	in.uniform = (SRPUniform*) P->uniform;
@@ -510,7 +524,8 @@ __device__ __forceinline__ bool insideTri(const SrpTri& T, const float* l)
 }
 
 template <class FS, bool LATE, bool WORDS>
-__global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
+__device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, const SrpDrawParams* P, uint32_t tile,
+                                                  uint32_t gtile)
 {
 	__shared__ SrpTri recs[64];
 	__shared__ unsigned long long triMask[64];
@@ -522,12 +537,8 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 	__shared__ float fragDep[RF];
 	__shared__ uint8_t fragState[RF];
 
-	const SrpDrawParams* P = L.params;
 	const uint32_t lane = threadIdx.x;
-	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
-	if (tile >= L.nTiles)
-		return;
-	const uint32_t begin = L.tileStart[tile], end = L.tileEnd[tile];
+	const uint32_t begin = L.tileStart[gtile], end = L.tileEnd[gtile];
 	if (begin == end)
 		return;
 	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;
@@ -561,7 +572,8 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 			const uint32_t pr = L.pairRec[pi];
 			known = (pr >> 31) != 0;
 			mask = known ? L.pairMask[pi] : 0ull;
-			recs[lane] = L.tris[pr & 0x7FFFFFFFu];
+			if (pr != 0xFFFFFFFFu)   /* a pair without a record (capacity exceeded) covers nothing */
+				recs[lane] = L.tris[pr & 0x7FFFFFFFu];
 		}
 		__syncthreads();
 		/* Triangles without an exact mask: pixel-parallel coverage. */
@@ -781,6 +793,53 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 	flushStats(P, covered, shaded);
 }
 
+template <class FS, bool LATE, bool WORDS>
+__global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
+{
+	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
+	if (tile >= L.nTiles)
+		return;
+	rasterTileBatched<FS, LATE, WORDS>(L, L.params, tile, tile);
+}
+
+/* Persistent multi-draw rasteriser: L.nBlocks waves take tiles of all the
+ * draws' framebuffers from device queues until none is left (every wave
+ * ends when it finds all queues drained).  With xcdQueues the tiles are
+ * split into 8 contiguous ranges, one per XCD (blocks are dispatched to
+ * XCDs round-robin), and a wave that drains its own range moves on to the
+ * next; otherwise one queue serves all. */
+template <class FS, bool LATE, bool WORDS>
+__global__ __launch_bounds__(64) void srp_raster_persistent(SrpRasterLaunch L)
+{
+	const uint32_t nq = L.xcdQueues ? 8u : 1u;
+	const uint32_t q0 = blockIdx.x % nq;
+	for (uint32_t k = 0; k < nq; k++)
+	{
+		const uint32_t q = (q0 + k) % nq;
+		const uint32_t lo = (uint32_t) (((uint64_t) L.nTiles * q) / nq);
+		const uint32_t hi = (uint32_t) (((uint64_t) L.nTiles * (q + 1u)) / nq);
+		for (;;)
+		{
+			uint32_t t = 0;
+			if (threadIdx.x == 0)
+				t = atomicAdd(&L.queues[q], 1u);
+			t = __shfl(t, 0) + lo;
+			if (t >= hi)
+				break;
+			/* the draw owning global tile t */
+			uint32_t a = 0, b = L.nSeg;
+			while (b - a > 1)
+			{
+				const uint32_t m = (a + b) >> 1;
+				if (L.seg[m].first <= t) a = m;
+				else b = m;
+			}
+			const SrpSeg sg = L.seg[a];
+			rasterTileBatched<FS, LATE, WORDS>(L, sg.P, t - sg.first, t);
+		}
+	}
+}
+
 /* Lines and points: fragments sorted (stably) by pixel; the first thread
  * of each pixel's run applies them in primitive order. */
 template <class FS, bool LATE, bool WORDS>
@@ -858,7 +917,7 @@ __global__ __launch_bounds__(256) void srp_frag_kernel(SrpFragLaunch L)
 template <class VS>
 void launchVS(const SrpVsLaunch* L, void* stream)
 {
-	const uint32_t blocks = (L->count + 255u) / 256u;
+	const uint32_t blocks = L->nSeg ? L->nBlocks : (L->count + 255u) / 256u;
 	if (blocks)
 		hipLaunchKernelGGL(srp_vs_kernel<VS>, dim3(blocks), dim3(256), 0, (hipStream_t) stream, *L);
 }
@@ -875,7 +934,10 @@ void launchRasterT(const SrpRasterLaunch* L, hipStream_t s)
 template <class FS, bool LATE, bool WORDS>
 void launchBatchedT(const SrpRasterLaunch* L, hipStream_t s)
 {
-	hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+	if (L->nSeg)
+		hipLaunchKernelGGL((srp_raster_persistent<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+	else
+		hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(64), 0, s, *L);
 }
 
 template <class FS>

@@ -10,13 +10,20 @@
  *  - textures from memory;
  *  - fragment/primitive counters and a stream handle for timing;
  *  - registration of HIP device shaders (used by the SRP_HIP_*_SHADER
- *    macros of srp/srp_device.h; not called directly by users). */
+ *    macros of srp/srp_device.h; not called directly by users);
+ *  - batches: many independent triangle draws (each with its own
+ *    framebuffer, buffers, uniform and render state) rendered together,
+ *    with one persistent rasteriser launch for all their tiles, optionally
+ *    captured once into a HIP graph and replayed, and tuned by measurement. */
 #pragma once
 
 #include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
+#include "srp/buffer.h"
+#include "srp/context.h"
 #include "srp/framebuffer.h"
+#include "srp/shaders.h"
 #include "srp/texture.h"
 #include "srp/detail/qualifiers.h"
 
@@ -66,5 +73,58 @@ int srpHipGetTileSize(void);
  *  struct SrpVsEntry / SrpFsEntry of srp/device/abi.h. */
 void srpHipRegisterVertexShader(const void* hostFunction, const void* entry);
 void srpHipRegisterFragmentShader(const void* hostFunction, const void* entry);
+
+/* ------------------------------------------------------------------ batches */
+
+/** One draw of a batch: the arguments of srpDrawIndexBuffer (ib NULL:
+ *  srpDrawVertexBuffer) plus its render state and an optional clear of its
+ *  framebuffer at the start of every frame (colour 0, depth clearDepth,
+ *  stencil clearStencil).  Framebuffers must be in deferred mode; draws of
+ *  a batch share one vertex and one fragment shader; primitives are
+ *  SRP_PRIM_TRIANGLES / _STRIP / _FAN in SRP_POLYGON_MODE_FILL. */
+typedef struct SRPDrawCommand
+{
+	const SRPIndexBuffer* ib;
+	const SRPVertexBuffer* vb;
+	const SRPFramebuffer* fb;
+	const SRPShaderProgram* sp;
+	SRPPrimitive primitive;
+	size_t startIndex;
+	size_t count;
+	const SRPContext* context;   /**< NULL: the current context when the batch is made */
+	int clear;
+	float clearDepth;
+	uint8_t clearStencil;
+} SRPDrawCommand;
+
+/** Implementation choices of a batch; none changes the rendered result. */
+typedef struct SRPBatchTuning
+{
+	int wavesPerCU;   /**< persistent rasteriser waves per compute unit (1..32) */
+	int maskBox;      /**< exact 8x8 binning masks for triangle boxes up to this side (0..24 px) */
+	int xcdQueues;    /**< 1: one tile queue per XCD range (work stealing between them); 0: one queue */
+} SRPBatchTuning;
+
+typedef struct SRPBatch SRPBatch;
+
+/** Validate and prepare the draws (device memory is sized by one counting
+ *  pass over them).  NULL, with a message, on invalid input.  The draws'
+ *  uniforms, vertex and index data are read when the batch is made and by
+ *  every frame; the commands themselves are copied. */
+SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n);
+void srpFreeBatch(SRPBatch* batch);
+/** Render one frame of every draw in command order (a replay of the HIP
+ *  graph once captured).  Queued on the library stream. */
+void srpBatchRender(SRPBatch* batch);
+/** Capture the frame into a HIP graph; later frames replay it.  0 on success. */
+int srpBatchCapture(SRPBatch* batch);
+/** Change the tuning (drops a captured graph) / read it. */
+void srpBatchSetTuning(SRPBatch* batch, const SRPBatchTuning* tuning);
+void srpBatchGetTuning(const SRPBatch* batch, SRPBatchTuning* tuning);
+/** Time `trials` frames of every candidate tuning and keep the fastest;
+ *  returns its mean frame time in ms and writes it to *best (if not NULL). */
+double srpBatchAutotune(SRPBatch* batch, int trials, SRPBatchTuning* best);
+/** draws, assembled primitives, binned (tile, triangle) pairs, tiles of the last frame sizing */
+void srpBatchInfo(const SRPBatch* batch, uint64_t out[4]);
 
 SRP_EXTERN_C_END

@@ -33,6 +33,9 @@ HIP_FLAGS = [
     "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument", "-Wno-pass-failed",
+    # host code: automatic variables start zeroed (launch-argument structs
+    # handed to kernels can never carry stack garbage)
+    "-Xarch_host", "-ftrivial-auto-var-init=zero",
 ] + NUMERIC_FLAGS
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-Wall", "-Wno-unused-function"] + NUMERIC_FLAGS
 
@@ -105,7 +108,7 @@ def build_scenes(jobs=8):
     host = os.path.join(CSRC, "host")
     return build_shader_module(
         [os.path.join(SCENE_DIR, "device_shaders.hip"), os.path.join(SCENE_DIR, "copy_probe.hip")],
-        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c", "bench.c", "messages.c", "terrain.c")] + [os.path.join(host, "obj_load.c")],
+        [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c", "bench.c", "messages.c", "terrain.c", "batch.c")] + [os.path.join(host, "obj_load.c")],
         SCENE_LIB, includes=[SCENE_DIR, host], jobs=jobs, defines=["SRP_GPU_BUILD"])
 
 

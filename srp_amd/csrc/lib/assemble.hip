@@ -83,10 +83,24 @@ __device__ __forceinline__ uint64_t loadIndex(const uint8_t* ib, uint32_t elem, 
 	}
 }
 
-__device__ __forceinline__ uint32_t slotOf(const AsmArgs& A, uint64_t stream)
+__device__ __forceinline__ uint32_t slotOf(const AsmDraw& W, uint64_t stream)
 {
-	const uint64_t vi = A.indexElem ? loadIndex(A.indices, A.indexElem, stream) : stream;
-	return (uint32_t) (vi - A.P->cacheBase);
+	const uint64_t vi = W.indexElem ? loadIndex(W.indices, W.indexElem, stream) : stream;
+	return (uint32_t) (vi - W.P->cacheBase);
+}
+
+/* The draw a block of a (possibly multi-draw) assembly launch works on:
+ * the last draw whose first block is at or before it. */
+__device__ __forceinline__ uint32_t drawOfBlock(const AsmDraw* D, uint32_t n, uint32_t block)
+{
+	uint32_t lo = 0, hi = n;
+	while (hi - lo > 1)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (D[mid].firstBlock <= block) lo = mid;
+		else hi = mid;
+	}
+	return lo;
 }
 
 __device__ __forceinline__ void loadClip(const SrpDrawParams* P, uint32_t slot, float* c)
@@ -345,14 +359,14 @@ __device__ bool setupTriangle(const SrpDrawParams* P, Vtx v0, Vtx v1, Vtx v2, ui
 }
 
 /* Tiles a triangle is binned to.  A triangle whose bounding box is at
- * most SRP_SMALL_BBOX pixels on each side is scanned here exactly as the
+ * most P->maskBox (<= SRP_SMALL_BBOX) pixels on each side is scanned here exactly as the
  * reference scans it (raster/triangle.c:78-110: rows from the box origin,
  * dldy per row, dldx per pixel), giving one 64-bit coverage mask per 8x8
  * tile; tiles it does not cover get no entry.  Larger triangles are binned
  * to every tile of their box and their coverage is found by the rasteriser.
  * Calls emit(tileKey, mask, known) in row-major tile order. */
 template <class F>
-__device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const SrpTri& T, F&& emit)
+__device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const SrpTri& T, uint32_t tileBase, F&& emit)
 {
 	if (T.x1 <= T.x0 || T.y1 <= T.y0)
 		return 0;
@@ -361,45 +375,56 @@ __device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const Sr
 	const uint32_t ty0 = T.y0 >> sh, ty1 = (T.y1 - 1u) >> sh;
 	const int bw = (int) T.x1 - (int) T.x0, bh = (int) T.y1 - (int) T.y0;
 	uint32_t n = 0;
-	if (sh == 3 && bw <= SRP_SMALL_BBOX && bh <= SRP_SMALL_BBOX)
+	if (sh == 3 && bw <= (int) P->maskBox && bh <= (int) P->maskBox)
 	{
-		uint64_t mask[4][4];
-		for (int a = 0; a < 4; a++)
-			for (int b = 0; b < 4; b++) mask[a][b] = 0;
+		/* One band of tile rows at a time; the (at most four) tiles of a
+		 * band keep their masks in registers.  The walk itself is the
+		 * reference's row-major scan of the whole box. */
 		float row[3] = {T.l0[0], T.l0[1], T.l0[2]};
-		for (int y = T.y0; y < (int) T.y1; y++)
-		{
-			float l[3] = {row[0], row[1], row[2]};
-			for (int x = T.x0; x < (int) T.x1; x++)
-			{
-				bool inside = true;
-				for (int e = 0; e < 3; e++)
-				{
-					const bool tl = (T.flags >> e) & 1u;
-					inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
-				}
-				if (inside)
-					mask[(y >> 3) - (int) ty0][(x >> 3) - (int) tx0] |= 1ull << (((y & 7) << 3) | (x & 7));
-				for (int e = 0; e < 3; e++) l[e] += T.dx[e];
-			}
-			for (int e = 0; e < 3; e++) row[e] += T.dy[e];
-		}
+		int y = T.y0;
 		for (uint32_t ty = ty0; ty <= ty1; ty++)
-			for (uint32_t tx = tx0; tx <= tx1; tx++)
+		{
+			uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+			const int yEnd = ((int) (ty + 1u) << 3) < (int) T.y1 ? (int) (ty + 1u) << 3 : (int) T.y1;
+			for (; y < yEnd; y++)
 			{
-				const uint64_t m = mask[ty - ty0][tx - tx0];
-				if (m)
+				float l[3] = {row[0], row[1], row[2]};
+				for (int x = T.x0; x < (int) T.x1; x++)
 				{
-					emit(ty * P->tilesX + tx, m, true);
+					bool inside = true;
+					for (int e = 0; e < 3; e++)
+					{
+						const bool tl = (T.flags >> e) & 1u;
+						inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
+					}
+					if (inside)
+					{
+						const uint32_t c = ((uint32_t) x >> 3) - tx0;
+						const uint64_t bit = 1ull << ((((uint32_t) y & 7u) << 3) | ((uint32_t) x & 7u));
+						m0 |= c == 0 ? bit : 0ull;
+						m1 |= c == 1 ? bit : 0ull;
+						m2 |= c == 2 ? bit : 0ull;
+						m3 |= c == 3 ? bit : 0ull;
+					}
+					for (int e = 0; e < 3; e++) l[e] += T.dx[e];
+				}
+				for (int e = 0; e < 3; e++) row[e] += T.dy[e];
+			}
+			const uint64_t ms[4] = {m0, m1, m2, m3};
+#pragma unroll
+			for (uint32_t c = 0; c < 4; c++)
+				if (tx0 + c <= tx1 && ms[c])
+				{
+					emit(tileBase + ty * P->tilesX + tx0 + c, ms[c], true);
 					n++;
 				}
-			}
+		}
 		return n;
 	}
 	for (uint32_t ty = ty0; ty <= ty1; ty++)
 		for (uint32_t tx = tx0; tx <= tx1; tx++)
 		{
-			emit(ty * P->tilesX + tx, ~0ull, false);
+			emit(tileBase + ty * P->tilesX + tx, ~0ull, false);
 			n++;
 		}
 	return n;
@@ -441,7 +466,7 @@ __device__ __forceinline__ uint32_t pixelKey(const SrpDrawParams* P, int px, int
 }
 
 /* line.c:48-76: the DDA walks x, y, t by repeated float addition. */
-__device__ void emitLineFrags(const AsmArgs& A, const SrpLine& L, uint32_t rec, uint32_t base)
+__device__ void emitLineFrags(const AsmArgs& A, const SrpDrawParams* P, const SrpLine& L, uint32_t rec, uint32_t base)
 {
 	const int steps = lineSteps(L);
 	if (steps < 0) return;
@@ -455,14 +480,14 @@ __device__ void emitLineFrags(const AsmArgs& A, const SrpLine& L, uint32_t rec, 
 		const int px = cvtI32(round((double) x)), py = cvtI32(round((double) y));
 		SrpFrag f;
 		f.px = px; f.py = py; f.prim = rec; f.t = t;
-		const uint32_t key = pixelKey(A.P, px, py);
+		const uint32_t key = pixelKey(P, px, py);
 		invalid += key == 0xFFFFFFFFu;
 		A.frags[base + i] = f;
 		A.fragKeys[base + i] = key;
 		x += xInc; y += yInc; t += tInc;
 	}
 	if (invalid)
-		atomicAdd(&A.P->stats[0], (unsigned long long) invalid);
+		atomicAdd(&P->stats[0], (unsigned long long) invalid);
 }
 
 /* point.c:81-118: pixel bounds of a point; false when fully outside. */
@@ -492,10 +517,10 @@ __device__ __forceinline__ bool pointCovers(const PointBox& b, int x, int y)
 }
 
 template <bool EMIT>
-__device__ uint32_t pointFrags(const AsmArgs& A, const SrpPoint& pt, uint32_t rec, uint32_t base)
+__device__ uint32_t pointFrags(const AsmArgs& A, const SrpDrawParams* P, const SrpPoint& pt, uint32_t rec, uint32_t base)
 {
 	PointBox b;
-	if (!pointBox(A.P, pt, b))
+	if (!pointBox(P, pt, b))
 		return 0;
 	uint32_t n = 0;
 	for (int y = b.y0; y <= b.y1; y++)
@@ -507,7 +532,7 @@ __device__ uint32_t pointFrags(const AsmArgs& A, const SrpPoint& pt, uint32_t re
 					SrpFrag f;
 					f.px = x; f.py = y; f.prim = rec; f.t = 0.0f;
 					A.frags[base + n] = f;
-					A.fragKeys[base + n] = (uint32_t) y * A.P->width + (uint32_t) x;
+					A.fragKeys[base + n] = (uint32_t) y * P->width + (uint32_t) x;
 				}
 				n++;
 			}
@@ -543,9 +568,11 @@ __device__ __forceinline__ void triStream(uint32_t prim, uint64_t base, uint64_t
 /* Assembled primitives of one clipped polygon v[0..n-1] (fan from v[0],
  * clipping.c:107-116), per polygon mode (primitive_assembly.c:80-128). */
 template <bool EMIT, int NMAX>
-__device__ __forceinline__ void emitFan(const AsmArgs& A, const SrpDrawParams* P, const Vtx* v, int n,
-                                        uint32_t recBase, uint32_t itemBase, uint32_t& nRec, uint32_t& nItems)
+__device__ __forceinline__ void emitFan(const AsmArgs& A, const AsmDraw& W, const Vtx* v, int n, uint32_t recBase,
+                                        uint32_t idBase, uint32_t itemBase, uint32_t& nRec, uint32_t& nItems,
+                                        bool drop = false)
 {
+	const SrpDrawParams* P = W.P;
 	const uint32_t mode = P->st.polygonMode;
 #pragma unroll
 	for (int f = 1; f + 1 < NMAX; f++)
@@ -558,24 +585,36 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const SrpDrawParams* P
 		if (mode == SRP_POLYGON_MODE_FILL)
 		{
 			SrpTri T;
-			if (!setupTriangle(P, a, b, c, recBase + nRec, T))
+			if (!setupTriangle(P, a, b, c, recBase + nRec - idBase, T))
 				continue;
 			const uint32_t r = recBase + nRec;
 			uint32_t np;
 			if (EMIT)
 			{
-				A.tris[r] = T;
+				/* A triangle beyond the record capacity (or with a clip vertex
+				 * beyond its capacity) keeps its counted pairs, written as
+				 * empty pairs without a record. */
+				const bool fits = r < A.recCap && !drop;
+				if (fits)
+					A.tris[r] = T;
+				else
+					atomicOr(A.error, 2u);
 				uint32_t o = itemBase + nItems;
-				np = binTriangle(P, T, [&](uint32_t key, uint64_t m, bool known) {
-					A.pairKeys[o] = key;
-					A.pairVals[o] = o;
-					A.pairRec[o] = r | (known ? 0x80000000u : 0u);
-					A.pairMask[o] = m;
+				np = binTriangle(P, T, W.tileBase, [&](uint32_t key, uint64_t m, bool known) {
+					if (o < A.itemCap)
+					{
+						A.pairKeys[o] = key;
+						A.pairVals[o] = o;
+						A.pairRec[o] = fits ? (r | (known ? 0x80000000u : 0u)) : 0xFFFFFFFFu;
+						A.pairMask[o] = fits ? m : 0ull;
+					}
 					o++;
 				});
+				if (o > A.itemCap)
+					atomicOr(A.error, 2u);
 			}
 			else
-				np = binTriangle(P, T, [](uint32_t, uint64_t, bool) {});
+				np = binTriangle(P, T, W.tileBase, [](uint32_t, uint64_t, bool) {});
 			nRec++;
 			nItems += np;
 		}
@@ -585,12 +624,12 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const SrpDrawParams* P
 			for (int j = 0; j < 3; j++)
 			{
 				SrpLine L;
-				setupLine(P, *tv[j], *tv[(j + 1) % 3], recBase + nRec, L);
+				setupLine(P, *tv[j], *tv[(j + 1) % 3], recBase + nRec - idBase, L);
 				const uint32_t nf = lineFragCount(L);
 				if (EMIT)
 				{
 					A.lines[recBase + nRec] = L;
-					emitLineFrags(A, L, recBase + nRec, itemBase + nItems);
+					emitLineFrags(A, P, L, recBase + nRec, itemBase + nItems);
 				}
 				nRec++;
 				nItems += nf;
@@ -602,9 +641,9 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const SrpDrawParams* P
 			for (int j = 0; j < 3; j++)
 			{
 				SrpPoint pt;
-				makePoint(P, *tv[j], recBase + nRec, pt);
+				makePoint(P, *tv[j], recBase + nRec - idBase, pt);
 				if (EMIT) A.points[recBase + nRec] = pt;
-				nItems += pointFrags<EMIT>(A, pt, recBase + nRec, itemBase + nItems);
+				nItems += pointFrags<EMIT>(A, P, pt, recBase + nRec, itemBase + nItems);
 				nRec++;
 			}
 		}
@@ -614,11 +653,11 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const SrpDrawParams* P
 /* A triangle that needs clipping: Sutherland-Hodgman over the pool, then
  * the varyings of the generated vertices that survive. */
 template <bool EMIT>
-__device__ __noinline__ void assembleClipped(const AsmArgs& A, const uint32_t* slot, uint32_t genBase,
-                                             uint32_t recBase, uint32_t itemBase,
+__device__ __noinline__ void assembleClipped(const AsmArgs& A, const AsmDraw& W, const uint32_t* slot, uint32_t genBase,
+                                             uint32_t recBase, uint32_t idBase, uint32_t itemBase,
                                              uint32_t& nRec, uint32_t& nGen, uint32_t& nItems)
 {
-	const SrpDrawParams* P = A.P;
+	const SrpDrawParams* P = W.P;
 	Pool pl;
 	for (int i = 0; i < 3; i++)
 	{
@@ -634,6 +673,7 @@ __device__ __noinline__ void assembleClipped(const AsmArgs& A, const uint32_t* s
 		n = 0;
 	}
 	Vtx v[SRP_MAX_POLY];
+	bool drop = false;
 	for (int i = 0; i < n; i++)
 	{
 		const int q = poly[i];
@@ -645,11 +685,16 @@ __device__ __noinline__ void assembleClipped(const AsmArgs& A, const uint32_t* s
 		{
 			v[i].vref = P->cacheSize + genBase + nGen;
 			if (EMIT)
-				materialize(P, pl, q, A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
+			{
+				if (genBase + nGen < A.genCap)
+					materialize(P, pl, q, A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
+				else
+					drop = true;
+			}
 			nGen++;
 		}
 	}
-	emitFan<EMIT, SRP_MAX_POLY>(A, P, v, n, recBase, itemBase, nRec, nItems);
+	emitFan<EMIT, SRP_MAX_POLY>(A, W, v, n, recBase, idBase, itemBase, nRec, nItems, drop);
 }
 
 /* Triangle draws (all polygon modes): primitive_assembly.c:37-135.  The
@@ -658,44 +703,48 @@ __device__ __noinline__ void assembleClipped(const AsmArgs& A, const uint32_t* s
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 {
-	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
-	if (k >= A.nIn)
+	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
+	const AsmDraw& W = A.draws[d];
+	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;
+	if (k >= W.nIn)
 		return;
-	const SrpDrawParams* P = A.P;
+	const uint32_t g = W.primBase + k;   /* slot of this input in the count arrays */
+	const SrpDrawParams* P = W.P;
 	uint64_t s[3];
-	triStream(A.prim, A.start, k, s);
+	triStream(W.prim, W.start, k, s);
 	uint32_t slot[3];
 	Vtx v[3];
 	uint32_t codeOr = 0, codeAnd = 0x3Fu;
 	for (int i = 0; i < 3; i++)
 	{
-		slot[i] = slotOf(A, s[i]);
+		slot[i] = slotOf(W, s[i]);
 		loadClip(P, slot[i], v[i].c);
 		const uint32_t c = outcode(v[i].c);
 		codeOr |= c;
 		codeAnd &= c;
 		v[i].vref = slot[i];
 	}
-	uint32_t recBase = 0, genBase = 0, itemBase = 0;
+	uint32_t recBase = 0, genBase = 0, itemBase = 0, idBase = 0;
 	if (EMIT)
 	{
-		recBase = A.recOff[k];
-		genBase = A.genOff[k];
-		itemBase = A.itemOff[k];
+		recBase = A.recOff[g];
+		genBase = A.genOff[g];
+		itemBase = A.itemOff[g];
+		idBase = A.recOff[W.primBase];
 	}
 	uint32_t nRec = 0, nGen = 0, nItems = 0;
 	if (codeOr == 0)
 	{
 		for (int i = 0; i < 3; i++) v[i].invW = divide(v[i].c);
-		emitFan<EMIT, 3>(A, P, v, 3, recBase, itemBase, nRec, nItems);
+		emitFan<EMIT, 3>(A, W, v, 3, recBase, idBase, itemBase, nRec, nItems);
 	}
 	else if (codeAnd == 0)
-		assembleClipped<EMIT>(A, slot, genBase, recBase, itemBase, nRec, nGen, nItems);
+		assembleClipped<EMIT>(A, W, slot, genBase, recBase, idBase, itemBase, nRec, nGen, nItems);
 	if (!EMIT)
 	{
-		A.recOff[k] = nRec;
-		A.genOff[k] = nGen;
-		A.itemOff[k] = nItems;
+		A.recOff[g] = nRec;
+		A.genOff[g] = nGen;
+		A.itemOff[g] = nItems;
 	}
 }
 
@@ -711,16 +760,19 @@ __device__ __forceinline__ void lineStream(uint32_t prim, uint64_t base, uint64_
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 {
-	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
-	if (k >= A.nIn)
+	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
+	const AsmDraw& W = A.draws[d];
+	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;
+	if (k >= W.nIn)
 		return;
-	const SrpDrawParams* P = A.P;
+	const uint32_t g = W.primBase + k;   /* slot of this input in the count arrays */
+	const SrpDrawParams* P = W.P;
 	Pool pl;
 	uint64_t s[2];
-	lineStream(A.prim, A.start, k, A.count, s);
+	lineStream(W.prim, W.start, k, W.count, s);
 	for (int i = 0; i < 2; i++)
 	{
-		pl.slot[i] = slotOf(A, s[i]);
+		pl.slot[i] = slotOf(W, s[i]);
 		loadClip(P, pl.slot[i], pl.c[i]);
 	}
 	pl.slot[2] = pl.slot[1];
@@ -763,9 +815,10 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 	uint32_t nRec = 0, nGen = 0, nItems = 0;
 	if (!rejected)
 	{
-		const uint32_t recBase = EMIT ? A.recOff[k] : 0u;
-		const uint32_t genBase = EMIT ? A.genOff[k] : 0u;
-		const uint32_t itemBase = EMIT ? A.itemOff[k] : 0u;
+		const uint32_t recBase = EMIT ? A.recOff[g] : 0u;
+		const uint32_t genBase = EMIT ? A.genOff[g] : 0u;
+		const uint32_t itemBase = EMIT ? A.itemOff[g] : 0u;
+		const uint32_t idBase = EMIT ? A.recOff[W.primBase] : 0u;
 		Vtx e[2];
 		const int q[2] = {ia, ib};
 		for (int i = 0; i < 2; i++)
@@ -783,20 +836,20 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 			}
 		}
 		SrpLine L;
-		setupLine(P, e[0], e[1], recBase, L);
+		setupLine(P, e[0], e[1], recBase - idBase, L);
 		nItems = lineFragCount(L);
 		if (EMIT)
 		{
 			A.lines[recBase] = L;
-			emitLineFrags(A, L, recBase, itemBase);
+			emitLineFrags(A, P, L, recBase, itemBase);
 		}
 		nRec = 1;
 	}
 	if (!EMIT)
 	{
-		A.recOff[k] = nRec;
-		A.genOff[k] = nGen;
-		A.itemOff[k] = nItems;
+		A.recOff[g] = nRec;
+		A.genOff[g] = nGen;
+		A.itemOff[g] = nItems;
 	}
 }
 
@@ -804,11 +857,14 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 {
-	const uint32_t k = blockIdx.x * 128u + threadIdx.x;
-	if (k >= A.nIn)
+	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
+	const AsmDraw& W = A.draws[d];
+	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;
+	if (k >= W.nIn)
 		return;
-	const SrpDrawParams* P = A.P;
-	const uint32_t slot = slotOf(A, A.start + k);
+	const uint32_t g = W.primBase + k;   /* slot of this input in the count arrays */
+	const SrpDrawParams* P = W.P;
+	const uint32_t slot = slotOf(W, W.start + k);
 	float c[4];
 	loadClip(P, slot, c);
 	const float x = c[0], y = c[1], z = c[2], w = c[3];
@@ -816,29 +872,30 @@ __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 	uint32_t nRec = 0, nItems = 0;
 	if (!clipped)
 	{
-		const uint32_t recBase = EMIT ? A.recOff[k] : 0u;
-		const uint32_t itemBase = EMIT ? A.itemOff[k] : 0u;
+		const uint32_t recBase = EMIT ? A.recOff[g] : 0u;
+		const uint32_t itemBase = EMIT ? A.itemOff[g] : 0u;
+		const uint32_t idBase = EMIT ? A.recOff[W.primBase] : 0u;
 		Vtx v;
 		for (int i = 0; i < 4; i++) v.c[i] = c[i];
 		v.invW = divide(v.c);
 		v.vref = slot;
 		SrpPoint pt;
-		makePoint(P, v, recBase, pt);
+		makePoint(P, v, recBase - idBase, pt);
 		if (EMIT) A.points[recBase] = pt;
-		nItems = pointFrags<EMIT>(A, pt, recBase, itemBase);
+		nItems = pointFrags<EMIT>(A, P, pt, recBase, itemBase);
 		nRec = 1;
 	}
 	if (!EMIT)
 	{
-		A.recOff[k] = nRec;
-		A.genOff[k] = 0;
-		A.itemOff[k] = nItems;
+		A.recOff[g] = nRec;
+		A.genOff[g] = 0;
+		A.itemOff[g] = nItems;
 	}
 }
 
 void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s)
 {
-	const uint32_t blocks = (A.nIn + 127u) / 128u;
+	const uint32_t blocks = A.nBlocks;
 	if (!blocks) return;
 	switch (kind)
 	{

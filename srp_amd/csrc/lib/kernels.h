@@ -12,21 +12,35 @@
 
 #define SRP_POOL 32       /* clip pool: 3 originals + generated vertices */
 #define SRP_MAX_POLY 16   /* vertices of a clipped polygon */
-#define SRP_SMALL_BBOX 24 /* triangles up to this box side get exact tile masks when binned */
+#define SRP_SMALL_BBOX 24 /* largest box side (P->maskBox) for exact tile masks at binning */
 
 namespace srp {
 namespace lib {
 
+/* One draw of an assembly launch (device memory).  A launch covers one
+ * draw or a batch of them; its blocks are split between the draws by
+ * firstBlock, and the draws' inputs are laid out one after another in the
+ * count arrays from primBase on. */
+struct AsmDraw
+{
+	const SrpDrawParams* P;
+	const uint8_t* indices;
+	uint64_t start;       /* first stream index */
+	uint64_t count;       /* stream indices of the draw (line loops wrap) */
+	uint32_t prim;        /* SRPPrimitive */
+	uint32_t nIn;         /* input primitives */
+	uint32_t indexElem;   /* 0: non-indexed, else bytes per index */
+	uint32_t primBase;    /* first slot of the draw in recOff/genOff/itemOff */
+	uint32_t tileBase;    /* first tile of the draw's framebuffer in the tile keys */
+	uint32_t firstBlock;  /* first block of the draw in the launch */
+};
+
 /* Arguments of the assembly kernels (assemble.hip). */
 struct AsmArgs
 {
-	const SrpDrawParams* P;
-	uint32_t prim;        /* SRPPrimitive */
-	uint32_t nIn;         /* input primitives */
-	uint64_t start;       /* first stream index */
-	uint64_t count;       /* stream indices of the draw (line loops wrap) */
-	const uint8_t* indices;
-	uint32_t indexElem;   /* 0: non-indexed, else bytes per index */
+	const AsmDraw* draws; /* device */
+	uint32_t nDraws;
+	uint32_t nBlocks;     /* blocks of 128 threads over all draws */
 	/* COUNT pass: per-input counts; EMIT pass: their exclusive scans */
 	uint32_t* recOff;
 	uint32_t* genOff;
@@ -42,7 +56,10 @@ struct AsmArgs
 	SrpFrag* frags;
 	uint32_t* fragKeys;
 	uint8_t* genVaryings;
-	uint32_t* error;
+	uint32_t* error;      /* bit 0: clip overflow; bit 1: a capacity below was exceeded */
+	/* capacities of the triangle outputs (records, tile pairs, generated
+	 * vertices); what lies beyond them is dropped and flagged */
+	uint32_t recCap, itemCap, genCap;
 };
 
 /* kind: 0 triangles, 1 lines, 2 points */
@@ -59,8 +76,10 @@ void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scrat
 /* The same for nArrays arrays of n values `stride` apart; totals[a] = sums. */
 void scanExclusiveMulti(uint32_t* base, size_t stride, int nArrays, size_t n, uint32_t* totals,
                         ScanScratch& scratch, hipStream_t s);
-/* [start, end) of each tile's run in a tile-sorted key array. */
-void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s);
+/* [start, end) of each tile's run in a tile-sorted key array.  With nDev,
+ * n is a capacity and the count is min(n, *nDev), read on the device. */
+void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s,
+                const uint32_t* nDev = nullptr);
 
 struct SortScratch
 {
@@ -72,9 +91,10 @@ struct SortScratch
 };
 /* Stable LSD radix sort of (keys, vals) on the low `bits` bits of the keys.
  * The sorted arrays are returned through *outKeys / *outVals (either the
- * inputs or the scratch buffers). */
+ * inputs or the scratch buffers).  With nDev, n is a capacity and the count
+ * is min(n, *nDev), read on the device (no host synchronisation). */
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& scratch,
-               uint32_t** outKeys, uint32_t** outVals, hipStream_t s);
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr);
 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);

@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -91,7 +92,10 @@ struct DevBuf
 };
 
 /* Parameter block: SrpDrawParams, then the uniform snapshot. */
-constexpr size_t kParamsHead = (sizeof(SrpDrawParams) + 255) & ~(size_t) 255;
+/* Parameter block of a draw: SrpDrawParams, the draw's assembly record,
+ * then (256-aligned) the inline uniform. */
+constexpr size_t kAsmDrawOff = (sizeof(SrpDrawParams) + 63) & ~(size_t) 63;
+constexpr size_t kParamsHead = (kAsmDrawOff + sizeof(AsmDraw) + 255) & ~(size_t) 255;
 constexpr size_t kMaxInlineUniform = 16384;
 constexpr size_t kParamsBlock = kParamsHead + kMaxInlineUniform;
 
@@ -107,6 +111,7 @@ struct Runtime
 	uint64_t hostPrims = 0, hostDraws = 0;
 	int tile = 8;
 	int batched = 1;  /* three-phase batched rasteriser (tile 8) */
+	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
 
 	DevBuf uniform, clip, varyings, genVaryings, counts, totals;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount;
@@ -863,6 +868,80 @@ int bitsFor(uint64_t n)
 	return b;
 }
 
+/* Varying layout of a vertex shader (interpolation.c:93-163): offsets,
+ * the 4-byte word path when every element allows it.  False (with a
+ * message) when the layout exceeds the device limits. */
+bool varyingLayout(const SRPVertexShader* V, SrpDrawParams& H)
+{
+	uint32_t off = 0;
+	bool words = (V->varyingsSize % 4) == 0;
+	uint32_t nWords = 0;
+	for (size_t a = 0; a < V->nVaryings; a++)
+	{
+		const SRPVaryingInfo& vi = V->varyingsInfo[a];
+		const size_t es = elemSizeOf(vi.type);
+		if (es == 0)
+			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "interpolateAttributes", "Unexpected type (%i)", vi.type);
+		H.var[a].nItems = (uint32_t) vi.nItems;
+		H.var[a].type = vi.type;
+		H.var[a].mode = vi.interpolationMode;
+		H.var[a].offset = off;
+		const bool fourByte = vi.type == SRP_FLOAT || vi.type == SRP_INT32 || vi.type == SRP_UINT32;
+		if (!fourByte || (off % 4) != 0)
+			words = false;
+		for (size_t e = 0; e < vi.nItems && words; e++)
+		{
+			if (nWords >= SRP_MAX_WORDS) { words = false; break; }
+			uint8_t k = SRP_WORD_PROVOKING;
+			if (vi.type == SRP_FLOAT && vi.interpolationMode == SRP_INTERPOLATION_MODE_PERSPECTIVE) k = SRP_WORD_PERSPECTIVE;
+			else if (vi.type == SRP_FLOAT && vi.interpolationMode == SRP_INTERPOLATION_MODE_AFFINE) k = SRP_WORD_AFFINE;
+			H.wordKind[nWords++] = k;
+		}
+		off += (uint32_t) (es * vi.nItems);
+	}
+	if (off > SRP_MAX_VARYING_BYTES)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "Varyings larger than %d bytes are not supported",
+		        SRP_MAX_VARYING_BYTES);
+		return false;
+	}
+	H.nVaryings = (uint32_t) V->nVaryings;
+	H.describedSize = off;
+	H.varyingsSize = (uint32_t) V->varyingsSize;
+	H.wordPath = words ? 1u : 0u;
+	H.nWords = words ? nWords : 0u;
+	return true;
+}
+
+/* Render state captured when a draw is issued. */
+bool snapshotState(const SRPContext* C, const SRPShaderProgram* sp, int kind, SrpState& S)
+{
+	S.provokingLast = C->provokingVertexMode == SRP_PROVOKING_VERTEX_LAST;
+	S.cullFace = C->raster.cullFace;
+	S.frontFace = C->raster.frontFace;
+	S.polygonMode = C->raster.polygonMode;
+	if (kind == 0 && S.polygonMode > SRP_POLYGON_MODE_POINT)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "assembleTrianglesGeneric",
+		        "Unexpected srpContext.raster.polygonMode (%i)", C->raster.polygonMode);
+		return false;
+	}
+	S.pointSize = C->raster.pointSize;
+	S.scissorEnabled = C->scissor.enabled;
+	S.sx0 = C->scissor.x;
+	S.sx1 = C->scissor.x + C->scissor.width;
+	S.sy0 = C->scissor.y;
+	S.sy1 = C->scissor.y + C->scissor.height;
+	S.stencilEnabled = C->stencil.enabled;
+	S.stencil[0] = C->stencil.front;
+	S.stencil[1] = C->stencil.back;
+	S.depthTest = C->depth.testEnable;
+	S.depthWrite = C->depth.writeEnable;
+	S.depthOp = C->depth.compareOp;
+	S.lateZ = sp->fs->mayOverwriteDepth;
+	return true;
+}
+
 void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFramebuffer* fb,
                 const SRPShaderProgram* sp, SRPPrimitive prim, size_t start, size_t count)
 {
@@ -956,43 +1035,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "At most %d varyings are supported", SRP_MAX_VARYINGS);
 		return;
 	}
-	uint32_t off = 0;
-	bool words = (V->varyingsSize % 4) == 0;
-	uint32_t nWords = 0;
-	for (size_t a = 0; a < V->nVaryings; a++)
-	{
-		const SRPVaryingInfo& vi = V->varyingsInfo[a];
-		const size_t es = elemSizeOf(vi.type);
-		if (es == 0)
-			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "interpolateAttributes", "Unexpected type (%i)", vi.type);
-		H.var[a].nItems = (uint32_t) vi.nItems;
-		H.var[a].type = vi.type;
-		H.var[a].mode = vi.interpolationMode;
-		H.var[a].offset = off;
-		const bool fourByte = vi.type == SRP_FLOAT || vi.type == SRP_INT32 || vi.type == SRP_UINT32;
-		if (!fourByte || (off % 4) != 0)
-			words = false;
-		for (size_t e = 0; e < vi.nItems && words; e++)
-		{
-			if (nWords >= SRP_MAX_WORDS) { words = false; break; }
-			uint8_t k = SRP_WORD_PROVOKING;
-			if (vi.type == SRP_FLOAT && vi.interpolationMode == SRP_INTERPOLATION_MODE_PERSPECTIVE) k = SRP_WORD_PERSPECTIVE;
-			else if (vi.type == SRP_FLOAT && vi.interpolationMode == SRP_INTERPOLATION_MODE_AFFINE) k = SRP_WORD_AFFINE;
-			H.wordKind[nWords++] = k;
-		}
-		off += (uint32_t) (es * vi.nItems);
-	}
-	if (off > SRP_MAX_VARYING_BYTES)
-	{
-		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "Varyings larger than %d bytes are not supported",
-		        SRP_MAX_VARYING_BYTES);
+	if (!varyingLayout(V, H))
 		return;
-	}
-	H.nVaryings = (uint32_t) V->nVaryings;
-	H.describedSize = off;
-	H.varyingsSize = (uint32_t) V->varyingsSize;
-	H.wordPath = words ? 1u : 0u;
-	H.nWords = words ? nWords : 0u;
 
 	/* Post-VS cache range (vertex_processing.c:23-36). */
 	uint64_t lo, hi;
@@ -1020,29 +1064,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 
 	/* State snapshot. */
 	SrpState& S = H.st;
-	S.provokingLast = C->provokingVertexMode == SRP_PROVOKING_VERTEX_LAST;
-	S.cullFace = C->raster.cullFace;
-	S.frontFace = C->raster.frontFace;
-	S.polygonMode = C->raster.polygonMode;
-	if (kind == 0 && S.polygonMode > SRP_POLYGON_MODE_POINT)
-	{
-		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "assembleTrianglesGeneric",
-		        "Unexpected srpContext.raster.polygonMode (%i)", C->raster.polygonMode);
+	if (!snapshotState(C, sp, kind, S))
 		return;
-	}
-	S.pointSize = C->raster.pointSize;
-	S.scissorEnabled = C->scissor.enabled;
-	S.sx0 = C->scissor.x;
-	S.sx1 = C->scissor.x + C->scissor.width;
-	S.sy0 = C->scissor.y;
-	S.sy1 = C->scissor.y + C->scissor.height;
-	S.stencilEnabled = C->stencil.enabled;
-	S.stencil[0] = C->stencil.front;
-	S.stencil[1] = C->stencil.back;
-	S.depthTest = C->depth.testEnable;
-	S.depthWrite = C->depth.writeEnable;
-	S.depthOp = C->depth.compareOp;
-	S.lateZ = sp->fs->mayOverwriteDepth;
 
 	H.width = (uint32_t) fb->width;
 	H.height = (uint32_t) fb->height;
@@ -1051,7 +1074,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.stencil = F->dStencil;
 
 	const size_t usize = vse->uniformSize > fse->uniformSize ? vse->uniformSize : fse->uniformSize;
-	size_t paramsBytes = sizeof(SrpDrawParams);
+	size_t paramsBytes = kAsmDrawOff + sizeof(AsmDraw);
 	if (sp->uniform && usize)
 	{
 		if (usize <= kMaxInlineUniform)
@@ -1085,6 +1108,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.varyings = R.varyings.get<uint8_t>((size_t) cacheSize * V->varyingsSize + 512);
 	H.genVaryings = H.varyings;
 	H.tileShift = (R.tile == 16) ? 4u : 3u;
+	H.maskBox = R.maskBox;
 	H.tilesX = (H.width + R.tile - 1) / R.tile;
 	H.tilesY = (H.height + R.tile - 1) / R.tile;
 	H.stats = R.dStats;
@@ -1094,10 +1118,24 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	else
 		fbFlushFills(F);
 	hipEvent_t evDraw = R.profile ? R.mark() : nullptr;
+	{
+		AsmDraw& AD = *reinterpret_cast<AsmDraw*>((uint8_t*) R.hParams + kAsmDrawOff);
+		AD.P = R.dParams;
+		AD.indices = ib ? ib->dData : nullptr;
+		AD.start = start;
+		AD.count = count;
+		AD.prim = (uint32_t) prim;
+		AD.nIn = (uint32_t) nIn;
+		AD.indexElem = ib ? (uint32_t) ib->elem : 0u;
+		AD.primBase = 0;
+		AD.tileBase = 0;
+		AD.firstBlock = 0;
+	}
 	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, paramsBytes, hipMemcpyHostToDevice, R.stream));
 
-	/* Vertex stage. */
+	/* Vertex stage (single draw: the multi-draw fields stay zero). */
 	SrpVsLaunch vl;
+	memset(&vl, 0, sizeof(vl));
 	vl.params = R.dParams;
 	vl.count = cacheSize;
 	vse->launch(&vl, (void*) R.stream);
@@ -1108,24 +1146,21 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	fillU32(totals, 0u, 8, R.stream);
 	AsmArgs A;
 	memset(&A, 0, sizeof(A));
-	A.P = R.dParams;
-	A.prim = prim;
-	A.nIn = (uint32_t) nIn;
-	A.start = start;
-	A.count = count;
-	A.indices = ib ? ib->dData : nullptr;
-	A.indexElem = ib ? (uint32_t) ib->elem : 0u;
+	A.draws = reinterpret_cast<const AsmDraw*>((const uint8_t*) R.dParams + kAsmDrawOff);
+	A.nDraws = 1;
+	A.nBlocks = (uint32_t) ((nIn + 127) / 128);
 	A.recOff = counts;
 	A.genOff = counts + (nIn + 1);
 	A.itemOff = counts + 2 * (nIn + 1);
 	A.error = totals + 3;
+	A.recCap = A.itemCap = A.genCap = 0xFFFFFFFFu;
 	launchAssemble(A, kind, false, R.stream);
 	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream);
 	if (R.profile) R.span(3, evDraw, R.mark());
 	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 	const uint32_t nRec = R.hTotals[0], nGen = R.hTotals[1], nItems = R.hTotals[2];
-	if (R.hTotals[3])
+	if (R.hTotals[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
 		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
 	R.hostDraws++;
@@ -1168,6 +1203,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		radixSort(A.pairKeys, A.pairVals, nItems, bitsFor(nTiles), R.sort, &keys, &vals, R.stream);
 		tileBounds(keys, nItems, tileStart, tileEnd, R.stream);
 		SrpRasterLaunch rl;
+		memset(&rl, 0, sizeof(rl));
 		rl.params = R.dParams;
 		rl.tris = A.tris;
 		rl.tileList = vals;
@@ -1193,6 +1229,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		uint32_t *keys, *vals;
 		radixSort(A.fragKeys, order, nItems, bitsFor((uint64_t) H.width * H.height + 1), R.sort, &keys, &vals, R.stream);
 		SrpFragLaunch fl;
+		memset(&fl, 0, sizeof(fl));
 		fl.params = R.dParams;
 		fl.frags = A.frags;
 		fl.keys = keys;
@@ -1229,6 +1266,552 @@ void srpDrawIndexBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, con
                         const SRPShaderProgram* sp, SRPPrimitive prim, size_t start, size_t count)
 {
 	drawBuffer(ib, vb, fb, sp, prim, start, count);
+}
+
+}  // extern "C"
+
+/* ====================================================================== batches
+ *
+ * A batch renders many independent triangle draws as one frame: one vertex
+ * launch over every draw's cache slots, one counting and one emitting
+ * assembly launch over every draw's primitives (blocks split by draw), one
+ * multi-array scan, one radix sort of the (tile, triangle) pairs of all
+ * framebuffers (tiles numbered globally), and one persistent rasteriser
+ * launch whose waves take tiles from device queues.  Buffers are sized by
+ * a counting pass when the batch is made, so a frame needs no host
+ * synchronisation and can be captured into a HIP graph. */
+
+struct SRPBatch
+{
+	std::vector<SRPDrawCommand> cmds;        /* drawn (not culled away) */
+	struct Clear { FbImpl* F; uint32_t depthBits; uint8_t stencil; };
+	std::vector<Clear> clears;
+	std::vector<FbImpl*> fbs;                /* every framebuffer touched */
+	const SrpVsEntry* vse = nullptr;
+	const SrpFsEntry* fse = nullptr;
+	const SRPShaderProgram* sp = nullptr;
+	size_t usize = 0, stride = 0;
+	uint8_t* hBlocks = nullptr;              /* pinned parameter blocks */
+	uint8_t* dBlocks = nullptr;
+	std::vector<AsmDraw> hDraws;
+	AsmDraw* dDraws = nullptr;
+	std::vector<SrpSeg> hVsSeg, hRasterSeg;
+	SrpSeg* dVsSeg = nullptr;
+	SrpSeg* dRasterSeg = nullptr;
+	uint32_t nIn = 0, inBlocks = 0, vsBlocks = 0, totalTiles = 0;
+	uint32_t lateZ = 0, words = 0;
+	float* clip = nullptr;
+	uint8_t* varyings = nullptr;
+	uint32_t* counts = nullptr;
+	uint32_t* totals = nullptr;
+	uint32_t* hTotals = nullptr;             /* pinned */
+	SrpTri* tris = nullptr;
+	uint32_t *pairKeys = nullptr, *pairVals = nullptr, *pairRec = nullptr;
+	unsigned long long* pairMask = nullptr;
+	uint8_t* genVaryings = nullptr;
+	uint32_t recCap = 0, itemCap = 0, genCap = 0;
+	uint32_t nRec = 0, nItems = 0, nGen = 0;
+	uint32_t* tileRange = nullptr;           /* start[totalTiles], end[totalTiles] */
+	uint32_t* queues = nullptr;
+	ScanScratch scan;
+	SortScratch sort;
+	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1};
+	int cus = 256;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void batchFreeDev(void*& p)
+{
+	if (p)
+	{
+		HIP_CHECK(hipFree(p));
+		p = nullptr;
+	}
+}
+
+void batchDropGraph(SRPBatch* B)
+{
+	if (B->exec) HIP_CHECK(hipGraphExecDestroy(B->exec));
+	if (B->graph) HIP_CHECK(hipGraphDestroy(B->graph));
+	B->exec = nullptr;
+	B->graph = nullptr;
+}
+
+SrpDrawParams* batchParams(SRPBatch* B, size_t d) { return (SrpDrawParams*) (B->hBlocks + d * B->stride); }
+
+void batchUpload(SRPBatch* B)
+{
+	Runtime& R = rt();
+	const size_t nd = B->cmds.size();
+	for (size_t d = 0; d < nd; d++)
+	{
+		batchParams(B, d)->maskBox = (uint32_t) B->tuning.maskBox;
+		batchParams(B, d)->genVaryings = B->genVaryings;
+	}
+	HIP_CHECK(hipMemcpyAsync(B->dBlocks, B->hBlocks, nd * B->stride, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(B->dDraws, B->hDraws.data(), nd * sizeof(AsmDraw), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(B->dVsSeg, B->hVsSeg.data(), nd * sizeof(SrpSeg), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(B->dRasterSeg, B->hRasterSeg.data(), nd * sizeof(SrpSeg), hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+}
+
+AsmArgs batchAsmArgs(SRPBatch* B)
+{
+	AsmArgs A;
+	memset(&A, 0, sizeof(A));
+	A.draws = B->dDraws;
+	A.nDraws = (uint32_t) B->cmds.size();
+	A.nBlocks = B->inBlocks;
+	A.recOff = B->counts;
+	A.genOff = B->counts + (B->nIn + 1);
+	A.itemOff = B->counts + 2 * ((size_t) B->nIn + 1);
+	A.tris = B->tris;
+	A.pairKeys = B->pairKeys;
+	A.pairVals = B->pairVals;
+	A.pairRec = B->pairRec;
+	A.pairMask = B->pairMask;
+	A.genVaryings = B->genVaryings;
+	A.error = B->totals + 3;
+	A.recCap = B->recCap;
+	A.itemCap = B->itemCap;
+	A.genCap = B->genCap;
+	return A;
+}
+
+/* Vertex stage + counting pass + scans (device totals in B->totals). */
+void batchFront(SRPBatch* B, hipStream_t s)
+{
+	SrpVsLaunch vl;
+	memset(&vl, 0, sizeof(vl));
+	vl.seg = B->dVsSeg;
+	vl.nSeg = (uint32_t) B->cmds.size();
+	vl.nBlocks = B->vsBlocks;
+	B->vse->launch(&vl, (void*) s);
+	fillU32(B->totals, 0u, 8, s);
+	AsmArgs A = batchAsmArgs(B);
+	launchAssemble(A, 0, false, s);
+	scanExclusiveMulti(A.recOff, (size_t) B->nIn + 1, 3, B->nIn, B->totals, B->scan, s);
+}
+
+/* Counting pass on the device, totals to the host; buffers (re)sized. */
+void batchSize(SRPBatch* B)
+{
+	Runtime& R = rt();
+	batchUpload(B);
+	batchFront(B, R.stream);
+	HIP_CHECK(hipMemcpyAsync(B->hTotals, B->totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	B->nRec = B->hTotals[0];
+	B->nGen = B->hTotals[1];
+	B->nItems = B->hTotals[2];
+	if (B->hTotals[3] & 1u)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
+		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
+	const uint32_t recCap = B->nRec + 64, genCap = B->nGen + 64, itemCap = B->nItems + 256;
+	if (recCap > B->recCap || genCap > B->genCap || itemCap > B->itemCap)
+	{
+		HIP_CHECK(hipDeviceSynchronize());
+		void* bufs[] = {B->tris, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings};
+		for (void* p : bufs)
+			if (p) HIP_CHECK(hipFree(p));
+		B->recCap = recCap + recCap / 8;
+		B->genCap = genCap + genCap / 8;
+		B->itemCap = itemCap + itemCap / 8;
+		HIP_CHECK(hipMalloc((void**) &B->tris, (size_t) B->recCap * sizeof(SrpTri)));
+		HIP_CHECK(hipMalloc((void**) &B->pairKeys, (size_t) B->itemCap * 4));
+		HIP_CHECK(hipMalloc((void**) &B->pairVals, (size_t) B->itemCap * 4));
+		HIP_CHECK(hipMalloc((void**) &B->pairRec, (size_t) B->itemCap * 4));
+		HIP_CHECK(hipMalloc((void**) &B->pairMask, (size_t) B->itemCap * 8));
+		HIP_CHECK(hipMalloc((void**) &B->genVaryings, (size_t) B->genCap * B->sp->vs->varyingsSize + 512));
+		batchUpload(B);
+	}
+}
+
+/* One frame of launches (capturable: fixed grids, no allocation, no host
+ * synchronisation, fills by kernels). */
+void batchLaunch(SRPBatch* B, hipStream_t s)
+{
+	for (const SRPBatch::Clear& c : B->clears)
+		fillFramebuffer(c.F->dColor, 0u, (uint32_t*) c.F->dDepth, c.depthBits, c.F->dStencil, c.stencil, c.F->pub.size, s);
+	batchFront(B, s);
+	AsmArgs A = batchAsmArgs(B);
+	launchAssemble(A, 0, true, s);
+	uint32_t* tileStart = B->tileRange;
+	uint32_t* tileEnd = B->tileRange + B->totalTiles;
+	fillU32(B->tileRange, 0u, (size_t) 2 * B->totalTiles, s);
+	uint32_t *keys, *vals;
+	radixSort(B->pairKeys, B->pairVals, B->itemCap, bitsFor(B->totalTiles), B->sort, &keys, &vals, s, B->totals + 2);
+	tileBounds(keys, B->itemCap, tileStart, tileEnd, s, B->totals + 2);
+	fillU32(B->queues, 0u, 8, s);
+	SrpRasterLaunch rl;
+	memset(&rl, 0, sizeof(rl));
+	rl.params = nullptr;
+	rl.tris = B->tris;
+	rl.tileList = vals;
+	rl.tileStart = tileStart;
+	rl.tileEnd = tileEnd;
+	rl.pairRec = B->pairRec;
+	rl.pairMask = B->pairMask;
+	rl.nTiles = B->totalTiles;
+	rl.lateZ = B->lateZ;
+	rl.words = B->words;
+	rl.tile = 8;
+	rl.batched = 1;
+	uint32_t waves = (uint32_t) (B->cus * B->tuning.wavesPerCU);
+	if (waves > B->totalTiles) waves = B->totalTiles;
+	rl.nBlocks = waves ? waves : 1u;
+	rl.seg = B->dRasterSeg;
+	rl.nSeg = (uint32_t) B->cmds.size();
+	rl.xcdQueues = B->tuning.xcdQueues ? 1u : 0u;
+	rl.queues = B->queues;
+	B->fse->raster(&rl, (void*) s);
+}
+
+/* Apply deferred fills recorded on the batch's framebuffers outside it. */
+void batchFlushPending(SRPBatch* B)
+{
+	for (FbImpl* F : B->fbs)
+		fbFlushFills(F);
+}
+
+}  // namespace
+
+extern "C" {
+
+SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
+{
+	if (!cmds || n == 0)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewBatch", "No draws");
+		return nullptr;
+	}
+	Runtime& R = rt();
+	SRPBatch* B = new SRPBatch();
+	auto fail = [&](const char* fmt, size_t i) {
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewBatch", fmt, i);
+		delete B;
+		return (SRPBatch*) nullptr;
+	};
+	struct Prep { uint64_t lo; uint32_t cacheSize, nIn; const SRPContext* C; };
+	std::vector<Prep> prep;
+	for (size_t i = 0; i < n; i++)
+	{
+		const SRPDrawCommand& c = cmds[i];
+		if (!c.vb || !c.fb || !c.sp || !c.sp->vs || !c.sp->fs)
+			return fail("Draw %zu: missing vertex buffer, framebuffer or shader program", i);
+		FbImpl* F = fbImpl(c.fb);
+		if (!F->deferred)
+			return fail("Draw %zu: batches draw into deferred framebuffers (srpFramebufferSetDeferred)", i);
+		if (c.fb->width > 65535 || c.fb->height > 65535 || c.fb->size >= 0xFFFFFFFFull)
+			return fail("Draw %zu: framebuffer too large for the GPU path", i);
+		if (std::find(B->fbs.begin(), B->fbs.end(), F) == B->fbs.end())
+			B->fbs.push_back(F);
+		if (c.clear)
+		{
+			uint32_t bits;
+			memcpy(&bits, &c.clearDepth, 4);
+			B->clears.push_back({F, bits, c.clearStencil});
+		}
+		if (c.primitive != SRP_PRIM_TRIANGLES && c.primitive != SRP_PRIM_TRIANGLE_STRIP &&
+		    c.primitive != SRP_PRIM_TRIANGLE_FAN)
+			return fail("Draw %zu: batches draw triangles only", i);
+		const SRPContext* C = c.context ? c.context : ctx();
+		if (C->raster.polygonMode != SRP_POLYGON_MODE_FILL)
+			return fail("Draw %zu: batches draw filled triangles only", i);
+		if (!B->sp)
+			B->sp = c.sp;
+		else if (c.sp->vs->shader != B->sp->vs->shader || c.sp->fs->shader != B->sp->fs->shader ||
+		         c.sp->fs->mayOverwriteDepth != B->sp->fs->mayOverwriteDepth ||
+		         c.sp->vs->varyingsSize != B->sp->vs->varyingsSize || c.sp->vs->nVaryings != B->sp->vs->nVaryings)
+			return fail("Draw %zu: the draws of a batch share one vertex and one fragment shader", i);
+		if (c.count == 0 || C->raster.cullFace == SRP_FACE_FRONT_AND_BACK)
+			continue;   /* draws nothing (pipeline/draw.c:63-65, :89-90) */
+		const size_t size = c.ib ? c.ib->nIndices : c.vb->nVertices;
+		if (c.startIndex + c.count - 1 >= size)
+			return fail("Draw %zu: indices out of bounds", i);
+		const size_t nIn = (c.primitive == SRP_PRIM_TRIANGLES) ? c.count / 3 : (c.count >= 3 ? c.count - 2 : 0);
+		if (nIn == 0)
+			continue;
+		uint64_t lo, hi;
+		if (c.ib)
+		{
+			indexRange(c.ib, c.startIndex, c.count, lo, hi);
+			if (hi >= c.vb->nVertices)
+				return fail("Draw %zu: an index exceeds the vertex buffer", i);
+		}
+		else
+		{
+			lo = c.startIndex;
+			hi = c.startIndex + c.count - 1;
+		}
+		if (hi - lo + 1 > 0x7FFFFFFFull || (uint64_t) B->nIn + nIn > 0x7FFFFFFFull)
+			return fail("Draw %zu: too many vertices or primitives", i);
+		B->cmds.push_back(c);
+		prep.push_back({lo, (uint32_t) (hi - lo + 1), (uint32_t) nIn, C});
+		B->nIn += (uint32_t) nIn;
+	}
+	{
+		std::lock_guard<std::mutex> g(g_regLock);
+		auto iv = vsRegistry().find((const void*) B->sp->vs->shader);
+		auto ifs = fsRegistry().find((const void*) B->sp->fs->shader);
+		if (iv != vsRegistry().end()) B->vse = iv->second;
+		if (ifs != fsRegistry().end()) B->fse = ifs->second;
+	}
+	if (!B->vse || !B->fse)
+		return fail("Draw %zu: a shader is not a registered device shader (see SRP_HIP_*_SHADER in srp/srp_device.h)",
+		            (size_t) 0);
+	const SRPVertexShader* V = B->sp->vs;
+	if (V->nVaryings > SRP_MAX_VARYINGS)
+		return fail("At most %zu varyings are supported", (size_t) SRP_MAX_VARYINGS);
+	HIP_CHECK(hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, 0));
+
+	const size_t nd = B->cmds.size();
+	B->usize = B->vse->uniformSize > B->fse->uniformSize ? B->vse->uniformSize : B->fse->uniformSize;
+	B->stride = kParamsHead + ((B->usize + 255) & ~(size_t) 255);
+	HIP_CHECK(hipHostMalloc((void**) &B->hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
+	HIP_CHECK(hipMalloc((void**) &B->totals, 8 * sizeof(uint32_t)));
+	if (nd == 0)
+		return B;   /* only clears */
+	HIP_CHECK(hipHostMalloc((void**) &B->hBlocks, nd * B->stride, hipHostMallocDefault));
+	memset(B->hBlocks, 0, nd * B->stride);
+	HIP_CHECK(hipMalloc((void**) &B->dBlocks, nd * B->stride));
+	size_t clipSlots = 0, varBytes = 0;
+	for (size_t d = 0; d < nd; d++)
+	{
+		clipSlots += prep[d].cacheSize;
+		varBytes += (size_t) prep[d].cacheSize * V->varyingsSize;
+	}
+	HIP_CHECK(hipMalloc((void**) &B->clip, clipSlots * 16 + 16));
+	HIP_CHECK(hipMalloc((void**) &B->varyings, varBytes + 512));
+	HIP_CHECK(hipMalloc((void**) &B->counts, ((size_t) B->nIn + 1) * 3 * sizeof(uint32_t)));
+	B->hDraws.resize(nd);
+	B->hVsSeg.resize(nd);
+	B->hRasterSeg.resize(nd);
+	size_t clipOff = 0, varOff = 0;
+	uint32_t primBase = 0, tileBase = 0, inBlock = 0, vsBlock = 0;
+	for (size_t d = 0; d < nd; d++)
+	{
+		const SRPDrawCommand& c = B->cmds[d];
+		SrpDrawParams& H = *batchParams(B, d);
+		const SrpDrawParams* dP = (const SrpDrawParams*) (B->dBlocks + d * B->stride);
+		if (!varyingLayout(V, H) || !snapshotState(prep[d].C, c.sp, 0, H.st))
+		{
+			srpFreeBatch(B);
+			return nullptr;
+		}
+		FbImpl* F = fbImpl(c.fb);
+		H.width = (uint32_t) c.fb->width;
+		H.height = (uint32_t) c.fb->height;
+		H.color = F->dColor;
+		H.depth = F->dDepth;
+		H.stencil = F->dStencil;
+		if (c.sp->uniform && B->usize)
+		{
+			memcpy((uint8_t*) &H + kParamsHead, c.sp->uniform, B->usize);
+			H.uniform = (const uint8_t*) dP + kParamsHead;
+		}
+		else
+			H.uniform = nullptr;
+		H.vertices = c.vb->dData;
+		H.vertexStride = c.vb->nBytesPerVertex;
+		H.cacheBase = prep[d].lo;
+		H.cacheSize = prep[d].cacheSize;
+		H.clip = B->clip + clipOff * 4;
+		H.varyings = B->varyings + varOff;
+		H.tileShift = 3u;
+		H.tilesX = (H.width + 7u) / 8u;
+		H.tilesY = (H.height + 7u) / 8u;
+		H.maskBox = (uint32_t) B->tuning.maskBox;
+		H.stats = R.dStats;
+		clipOff += prep[d].cacheSize;
+		varOff += (size_t) prep[d].cacheSize * V->varyingsSize;
+		B->lateZ = H.st.lateZ;
+		B->words = H.wordPath;
+
+		AsmDraw& AD = B->hDraws[d];
+		AD.P = dP;
+		AD.indices = c.ib ? c.ib->dData : nullptr;
+		AD.start = c.startIndex;
+		AD.count = c.count;
+		AD.prim = (uint32_t) c.primitive;
+		AD.nIn = prep[d].nIn;
+		AD.indexElem = c.ib ? (uint32_t) c.ib->elem : 0u;
+		AD.primBase = primBase;
+		AD.tileBase = tileBase;
+		AD.firstBlock = inBlock;
+		const uint32_t nTiles = H.tilesX * H.tilesY;
+		B->hVsSeg[d] = SrpSeg{dP, 0u, prep[d].cacheSize, vsBlock, 0u};
+		B->hRasterSeg[d] = SrpSeg{dP, tileBase, nTiles, 0u, 0u};
+		primBase += prep[d].nIn;
+		tileBase += nTiles;
+		inBlock += (prep[d].nIn + 127u) / 128u;
+		vsBlock += (prep[d].cacheSize + 255u) / 256u;
+	}
+	B->inBlocks = inBlock;
+	B->vsBlocks = vsBlock;
+	B->totalTiles = tileBase;
+	HIP_CHECK(hipMalloc((void**) &B->dDraws, nd * sizeof(AsmDraw)));
+	HIP_CHECK(hipMalloc((void**) &B->dVsSeg, nd * sizeof(SrpSeg)));
+	HIP_CHECK(hipMalloc((void**) &B->dRasterSeg, nd * sizeof(SrpSeg)));
+	HIP_CHECK(hipMalloc((void**) &B->tileRange, ((size_t) 2 * B->totalTiles + 2) * sizeof(uint32_t)));
+	HIP_CHECK(hipMalloc((void**) &B->queues, 8 * sizeof(uint32_t)));
+	batchSize(B);
+	/* one eager frame sizes the scan and sort scratch (none is allocated later) */
+	batchFlushPending(B);
+	batchLaunch(B, R.stream);
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	return B;
+}
+
+void srpFreeBatch(SRPBatch* B)
+{
+	if (!B)
+		return;
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+	batchDropGraph(B);
+	void* bufs[] = {B->dBlocks, B->dDraws, B->dVsSeg, B->dRasterSeg, B->clip, B->varyings, B->counts, B->totals,
+	                B->tris, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings, B->tileRange,
+	                B->queues, B->scan.buf, B->sort.keys, B->sort.vals, B->sort.hist, B->sort.scan.buf};
+	for (void* p : bufs)
+		batchFreeDev(p);
+	if (B->hBlocks) HIP_CHECK(hipHostFree(B->hBlocks));
+	if (B->hTotals) HIP_CHECK(hipHostFree(B->hTotals));
+	delete B;
+}
+
+void srpBatchRender(SRPBatch* B)
+{
+	if (!B)
+		return;
+	Runtime& R = rt();
+	batchFlushPending(B);
+	if (B->exec)
+	{
+		HIP_CHECK(hipGraphLaunch(B->exec, R.stream));
+		return;
+	}
+	if (B->cmds.empty())
+	{
+		for (const SRPBatch::Clear& c : B->clears)
+			fillFramebuffer(c.F->dColor, 0u, (uint32_t*) c.F->dDepth, c.depthBits, c.F->dStencil, c.stencil,
+			                c.F->pub.size, R.stream);
+		return;
+	}
+	hipEvent_t e0 = R.profile ? R.mark() : nullptr;
+	batchLaunch(B, R.stream);
+	if (R.profile) R.span(1, e0, R.mark());
+}
+
+int srpBatchCapture(SRPBatch* B)
+{
+	if (!B || B->cmds.empty())
+		return -1;
+	Runtime& R = rt();
+	batchDropGraph(B);
+	batchFlushPending(B);
+	/* an eager frame first: every scratch buffer has its final size */
+	batchLaunch(B, R.stream);
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	if (hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+		return -2;
+	batchLaunch(B, R.stream);
+	hipGraph_t g = nullptr;
+	if (hipStreamEndCapture(R.stream, &g) != hipSuccess || !g)
+		return -3;
+	if (hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0) != hipSuccess)
+	{
+		HIP_CHECK(hipGraphDestroy(g));
+		B->exec = nullptr;
+		return -4;
+	}
+	B->graph = g;
+	return 0;
+}
+
+void srpBatchSetTuning(SRPBatch* B, const SRPBatchTuning* t)
+{
+	if (!B || !t)
+		return;
+	if (t->wavesPerCU < 1 || t->wavesPerCU > 32 || t->maskBox < 0 || t->maskBox > SRP_SMALL_BBOX)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpBatchSetTuning",
+		        "Waves per CU must lie in 1..32 and the mask box in 0..%d", SRP_SMALL_BBOX);
+		return;
+	}
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+	batchDropGraph(B);
+	const bool resize = t->maskBox != B->tuning.maskBox;
+	B->tuning = *t;
+	if (!B->cmds.empty())
+	{
+		if (resize)
+			batchSize(B);   /* the mask box changes the number of tile pairs */
+		else
+			batchUpload(B);
+	}
+}
+
+void srpBatchGetTuning(const SRPBatch* B, SRPBatchTuning* t)
+{
+	if (B && t)
+		*t = B->tuning;
+}
+
+double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
+{
+	if (!B || B->cmds.empty())
+		return 0.0;
+	if (trials < 1) trials = 1;
+	Runtime& R = rt();
+	const int waves[] = {2, 4, 8, 16};
+	const int boxes[] = {0, 8, 16, 24};
+	const int queues[] = {0, 1};
+	SRPBatchTuning bestT = B->tuning;
+	double bestMs = 1e30;
+	hipEvent_t a, b;
+	HIP_CHECK(hipEventCreate(&a));
+	HIP_CHECK(hipEventCreate(&b));
+	for (int box : boxes)
+		for (int w : waves)
+			for (int q : queues)
+			{
+				SRPBatchTuning t = {w, box, q};
+				srpBatchSetTuning(B, &t);
+				batchFlushPending(B);
+				batchLaunch(B, R.stream);   /* warm-up */
+				HIP_CHECK(hipEventRecord(a, R.stream));
+				for (int k = 0; k < trials; k++)
+					batchLaunch(B, R.stream);
+				HIP_CHECK(hipEventRecord(b, R.stream));
+				HIP_CHECK(hipEventSynchronize(b));
+				float ms = 0.0f;
+				HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+				const double per = (double) ms / trials;
+				if (per < bestMs)
+				{
+					bestMs = per;
+					bestT = t;
+				}
+			}
+	HIP_CHECK(hipEventDestroy(a));
+	HIP_CHECK(hipEventDestroy(b));
+	srpBatchSetTuning(B, &bestT);
+	if (best)
+		*best = bestT;
+	return bestMs;
+}
+
+void srpBatchInfo(const SRPBatch* B, uint64_t out[4])
+{
+	if (!B || !out)
+		return;
+	out[0] = B->cmds.size();
+	out[1] = B->nRec;
+	out[2] = B->nItems;
+	out[3] = B->totalTiles;
 }
 
 }  // extern "C"

@@ -191,8 +191,9 @@ void scanExclusiveMulti(uint32_t* base, size_t stride, int nArrays, size_t n, ui
 
 /* Tile ranges of a list sorted by tile: [start, end) per tile; tiles that
  * received nothing keep start == end == 0 (set by the caller's fill). */
-__global__ void k_tile_bounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end)
+__global__ void k_tile_bounds(const uint32_t* keys, size_t n, const uint32_t* nDev, uint32_t* start, uint32_t* end)
 {
+	if (nDev && *nDev < n) n = *nDev;
 	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const uint32_t k = keys[i];
@@ -200,9 +201,9 @@ __global__ void k_tile_bounds(const uint32_t* keys, size_t n, uint32_t* start, u
 	if (i + 1 == n || keys[i + 1] != k) end[k] = (uint32_t) (i + 1);
 }
 
-void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s)
+void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s, const uint32_t* nDev)
 {
-	if (n) hipLaunchKernelGGL(k_tile_bounds, dim3((uint32_t) ((n + 255) / 256)), dim3(256), 0, s, keys, n, start, end);
+	if (n) hipLaunchKernelGGL(k_tile_bounds, dim3((uint32_t) ((n + 255) / 256)), dim3(256), 0, s, keys, n, nDev, start, end);
 }
 
 /* ------------------------------------------------------------ radix sort */
@@ -210,9 +211,10 @@ void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, 
 constexpr int RS_THREADS = 256;
 constexpr int RS_CHUNK = 4096;   /* items per block */
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys, size_t n, uint32_t shift,
-                                                           uint32_t* hist, uint32_t nBlocks)
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys, size_t n, const uint32_t* nDev,
+                                                           uint32_t shift, uint32_t* hist, uint32_t nBlocks)
 {
+	if (nDev && *nDev < n) n = *nDev;
 	__shared__ uint32_t h[256];
 	const uint32_t t = threadIdx.x;
 	h[t] = 0;
@@ -229,8 +231,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys,
  * of 256 items ranks its items per digit with wave ballots, then by wave. */
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
                                                               uint32_t* keysOut, uint32_t* valsOut, size_t n,
-                                                              uint32_t shift, const uint32_t* hist, uint32_t nBlocks)
+                                                              const uint32_t* nDev, uint32_t shift, const uint32_t* hist,
+                                                              uint32_t nBlocks)
 {
+	if (nDev && *nDev < n) n = *nDev;
 	__shared__ uint32_t running[256];
 	__shared__ uint32_t waveCnt[4][256];
 	__shared__ uint32_t tileTotal[256];
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 }
 
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& sc,
-               uint32_t** outKeys, uint32_t** outVals, hipStream_t s)
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev)
 {
 	*outKeys = keys;
 	*outVals = vals;
@@ -316,9 +320,10 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 	uint32_t* vOut = sc.vals;
 	for (int shift = 0; shift < bits; shift += 8)
 	{
-		hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, (uint32_t) shift, sc.hist, nBlocks);
+		hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift, sc.hist,
+		                   nBlocks);
 		scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
-		hipLaunchKernelGGL(k_radix_scatter, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
+		hipLaunchKernelGGL(k_radix_scatter, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n, nDev,
 		                   (uint32_t) shift, sc.hist, nBlocks);
 		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
 		uint32_t* tv = vIn; vIn = vOut; vOut = tv;
