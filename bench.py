@@ -220,6 +220,75 @@ def neural_leg(args):
     return out
 
 
+def batch_leg(args):
+    """The batched-scenes workload: 64 independent scenes rendered as one
+    batch (one persistent rasteriser launch for all their tiles), tuned by
+    the autotuner, captured into a HIP graph and replayed; compared with
+    drawing the scenes one at a time on the same GPU, and with the CPU
+    oracle (one frame)."""
+    from batch_runner import C4_SEED, BatchScenes, frames_equal
+    from neural_runner import Neural
+    nn = Neural()
+    gb = BatchScenes(GPU_SO, gpu=True)
+    gb.setup(args.batch_scenes, 1.0, C4_SEED, batched=True)
+    out = {}
+    try:
+        info = gb.info()
+        out["scenes"] = args.batch_scenes
+        out["triangles"] = int(sum(t for _, t in gb.scenes))
+        out["primitives_after_culling"] = int(info[1])
+        out["tile_pairs"] = int(info[2])
+        out["tiles"] = int(info[3])
+
+        def timed(fn, frames):
+            fn()
+            nn.sync()
+            nn.reset_stats()
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                fn()
+            nn.sync()
+            dt = (time.perf_counter() - t0) / frames
+            return dt * 1e3, nn.stats()[1] // frames
+
+        seq_ms, seq_shaded = timed(lambda: gb.frame(False), 3)
+        seq = gb.read_all()
+        eager_default_ms, _ = timed(lambda: gb.frame(True), 5)
+        t0 = time.perf_counter()
+        tuned_ms, tuning = gb.autotune(3)
+        out["autotune"] = {"s": round(time.perf_counter() - t0, 3), "best_ms": round(tuned_ms, 4),
+                           "waves_per_cu": tuning[0], "mask_box": tuning[1], "xcd_queues": tuning[2],
+                           "candidates": 32}
+        eager_ms, shaded = timed(lambda: gb.frame(True), 10)
+        assert gb.capture() == 0
+        graph_ms, g_shaded = timed(lambda: gb.frame(True), args.batch_frames)
+        got = gb.read_all()
+        out["frame_ms_graph"] = round(graph_ms, 4)
+        out["frame_ms_eager"] = round(eager_ms, 4)
+        out["frame_ms_eager_default_tuning"] = round(eager_default_ms, 4)
+        out["frame_ms_draw_by_draw"] = round(seq_ms, 4)
+        out["shaded_per_frame"] = int(g_shaded)
+        out["shaded_mfrags_per_s"] = round(g_shaded / (graph_ms * 1e-3) / 1e6, 2)
+        out["triangles_per_s"] = round(out["triangles"] / (graph_ms * 1e-3), 1)
+        out["parity_vs_draw_by_draw"] = frames_equal(got, seq) == [] and g_shaded == seq_shaded == shaded
+        if not args.no_cpu_baseline:
+            ob = BatchScenes(ORACLE_SO, gpu=False)
+            ob.setup(args.batch_scenes, 1.0, C4_SEED, batched=False)
+            try:
+                t0 = time.perf_counter()
+                ob.frame(False)
+                cpu_s = time.perf_counter() - t0
+                ref = ob.read_all()
+            finally:
+                ob.finish()
+            out["parity_vs_oracle"] = frames_equal(got, ref) == []
+            out["cpu_frame_s"] = round(cpu_s, 3)
+            out["speedup_vs_cpu"] = round(cpu_s * 1e3 / graph_ms, 1)
+    finally:
+        gb.finish()
+    return out
+
+
 def copy_probe(mib, reps=5):
     """Plain device copy of a known size: its FETCH_SIZE / WRITE_SIZE per
     dispatch calibrate the counters, its rate is the achievable copy BW."""
@@ -260,6 +329,10 @@ def main():
     ap.add_argument("--nn-frames", type=int, default=50)
     ap.add_argument("--neural-only", action="store_true", help="run only the neural-shader leg and print its JSON")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-shader leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-frames", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched-scenes leg and print its JSON")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates byte counters) and exit")
     args = ap.parse_args()
@@ -274,6 +347,9 @@ def main():
         return
     if args.neural_only:
         print(json.dumps({"neural": neural_leg(args)}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_leg(args)}))
         return
 
     tmp = tempfile.mkdtemp(prefix="srp_bench_")
@@ -383,6 +459,9 @@ def main():
         result["parity_vs_oracle"] = bool(same and cpu_shaded == shaded_per_frame)
         result["speedup_vs_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
 
+    if not args.no_batch and args.workload == "mesh":
+        print("bench.py: batched-scenes leg (64 scenes, autotune, graph)", file=sys.stderr, flush=True)
+        result["batch"] = batch_leg(args)
     if not args.no_neural and args.workload == "mesh":
         print("bench.py: neural shader leg (capture, training, evaluation)", file=sys.stderr, flush=True)
         result["neural"] = neural_leg(args)
