@@ -19,6 +19,7 @@
 
 #include "srp/shaders.h"
 #include "srp/device/abi.h"
+#include "srp/device/replay.h"
 
 #pragma clang fp contract(off)
 
@@ -85,9 +86,7 @@ __device__ __forceinline__ uint32_t packColor(const float c[4])
  * row (raster/triangle.c:101-109); `n` additions of `d` in that order. */
 __device__ __forceinline__ float replay(float a, float d, int n)
 {
-	for (int k = 0; k < n; k++)
-		a = a + d;
-	return a;
+	return srpReplay(a, d, n);
 }
 
 /* ---------------------------------------------------------------- varyings */
@@ -523,11 +522,66 @@ __device__ __forceinline__ bool insideTri(const SrpTri& T, const float* l)
 	return in;
 }
 
+/* Row bases of up to 8 triangles of a tile (the set bits of `round`, in
+ * order; bit b is the record in slot (head + b) % 64).  Lane (j = lane % 8, i = lane / 8) walks triangle j's
+ * barycentrics down to row i of the tile and along that row to the tile's
+ * first column inside the triangle's box, as the reference's scan would
+ * reach them; a pixel of the row then continues with at most 7 additions.
+ * (A walk split in two is the same sequence of additions.) */
+__device__ __forceinline__ void rowBases(const SrpTri* recs, uint32_t head, uint64_t round, uint32_t tx, uint32_t ty,
+                                         uint32_t lane, float (*base)[8][3])
+{
+	const uint32_t j = lane & 7u, i = lane >> 3;
+	uint64_t r = round;
+	for (uint32_t k = 0; k < j && r; k++)
+		r &= r - 1ull;
+	if (!r)
+		return;
+	const SrpTri& T = recs[(head + (uint32_t) __builtin_ctzll(r)) & 63u];
+	const int y = (int) (ty * 8u + i);
+	if (y < (int) T.y0 || y >= (int) T.y1)
+		return;
+	const int cs = ((int) (tx * 8u) > (int) T.x0) ? (int) (tx * 8u) : (int) T.x0;
+#pragma unroll
+	for (int e = 0; e < 3; e++)
+		base[j][i][e] = replay(replay(T.l0[e], T.dy[e], y - (int) T.y0), T.dx[e], cs - (int) T.x0);
+}
+
+/* Barycentrics of pixel (px, py) inside triangle slot j's box from its row base. */
+__device__ __forceinline__ void baryFromBase(const SrpTri& T, const float (*base)[8][3], uint32_t j, int px, int py,
+                                             uint32_t tx, float* l)
+{
+	const int cs = ((int) (tx * 8u) > (int) T.x0) ? (int) (tx * 8u) : (int) T.x0;
+	const int n = px - cs;
+#pragma unroll
+	for (int e = 0; e < 3; e++)
+	{
+		float v = base[j][py & 7][e];
+		for (int k = 0; k < n; k++)
+			v = v + T.dx[e];
+		l[e] = v;
+	}
+}
+
+/* The lowest (up to) 8 set bits of *rest, removed from it. */
+__device__ __forceinline__ uint64_t takeRound(uint64_t& rest)
+{
+	uint64_t round = 0, u = rest;
+	for (int k = 0; k < 8 && u; k++)
+	{
+		round |= u & (~u + 1ull);
+		u &= u - 1ull;
+	}
+	rest = u;
+	return round;
+}
+
 template <class FS, bool LATE, bool WORDS>
 __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, const SrpDrawParams* P, uint32_t tile,
                                                   uint32_t gtile)
 {
 	__shared__ SrpTri recs[64];
+	__shared__ float base[8][8][3];
 	__shared__ unsigned long long triMask[64];
 	__shared__ uint32_t fragBase[64];
 	__shared__ float fragL[3][RF];
@@ -560,40 +614,63 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	const uint32_t pv = st.provokingLast ? 2u : 0u;
 	const bool precull = !LATE && st.depthTest && monotoneOp(st.depthOp);
 
+	/* The tile's list is consumed through a window of 64 slots: slot s
+	 * (lane s keeps its mask, recs[s] its record) holds list entry
+	 * i + ((s - head) mod 64).  Each entry is loaded and its mask found once,
+	 * however many batches it waits before its fragments fit. */
+	uint32_t head = 0, filled = 0;
+	uint64_t smask = 0;
+	bool sknown = true;
 	for (uint32_t i = begin; i < end;)
 	{
-		/* A: masks.  Lane b loads pair i+b and its record. */
+		/* A: masks of the entries new to the window. */
 		const uint32_t avail = (end - i < 64u) ? end - i : 64u;
-		uint64_t mask = 0;
-		bool known = true;
-		if (lane < avail)
+		const uint32_t mine = (lane - head) & 63u;   /* this slot's position in the window */
+		const bool fresh = mine >= filled && mine < avail;
+		if (fresh)
 		{
-			const uint32_t pi = L.tileList[i + lane];
+			const uint32_t pi = L.tileList[i + mine];
 			const uint32_t pr = L.pairRec[pi];
-			known = (pr >> 31) != 0;
-			mask = known ? L.pairMask[pi] : 0ull;
+			sknown = (pr >> 31) != 0;
+			smask = sknown ? L.pairMask[pi] : 0ull;
 			if (pr != 0xFFFFFFFFu)   /* a pair without a record (capacity exceeded) covers nothing */
 				recs[lane] = L.tris[pr & 0x7FFFFFFFu];
 		}
 		__syncthreads();
-		/* Triangles without an exact mask: pixel-parallel coverage. */
-		uint64_t unknown = __ballot(lane < avail && !known);
+		/* Triangles without an exact mask: pixel-parallel coverage, 8
+		 * triangles per round from shared row bases (bits are slots). */
+		uint64_t unknown = __ballot(fresh && !sknown);
 		while (unknown)
 		{
-			const uint32_t b = (uint32_t) __builtin_ctzll(unknown);
-			unknown &= unknown - 1ull;
-			const SrpTri& T = recs[b];
-			bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
-			if (cov)
+			const uint64_t round = takeRound(unknown);
+			rowBases(recs, 0u, round, tx, ty, lane, base);
+			__syncthreads();
+			uint64_t r = round;
+			for (uint32_t j = 0; r; j++)
 			{
-				float l[3];
-				baryAt(T, px, py, l);
-				cov = insideTri(T, l);
+				const uint32_t b = (uint32_t) __builtin_ctzll(r);
+				r &= r - 1ull;
+				const SrpTri& T = recs[b];
+				bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
+				if (cov)
+				{
+					float l[3];
+					baryFromBase(T, base, j, px, py, tx, l);
+					cov = insideTri(T, l);
+				}
+				const uint64_t m = __ballot(cov);
+				if (lane == b)
+					smask = m;
 			}
-			const uint64_t m = __ballot(cov);
-			if (lane == b)
-				mask = m;
+			__syncthreads();
 		}
+		filled = avail;
+		/* Window order: lane b sees entry i + b. */
+		const uint32_t src = (head + lane) & 63u;
+		const uint32_t mlo = (uint32_t) __shfl((uint32_t) smask, (int) src);
+		const uint32_t mhi = (uint32_t) __shfl((uint32_t) (smask >> 32), (int) src);
+		const uint64_t mask = ((uint64_t) mhi << 32) | mlo;
+		const bool known = __shfl((int) sknown, (int) src) != 0;
 		/* Inclusive scan of fragment counts; keep the longest prefix that fits. */
 		const uint32_t cnt = (lane < avail) ? (uint32_t) __popcll(mask) : 0u;
 		uint32_t incl = cnt;
@@ -629,14 +706,20 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		uint64_t unk = __ballot(lane < nb && !known);
 		while (unk)
 		{
-			const uint32_t b = (uint32_t) __builtin_ctzll(unk);
-			unk &= unk - 1ull;
+			const uint64_t round = takeRound(unk);
+			rowBases(recs, head, round, tx, ty, lane, base);
+			__syncthreads();
+			uint64_t r = round;
+			for (uint32_t j = 0; r; j++)
+			{
+			const uint32_t b = (uint32_t) __builtin_ctzll(r);
+			r &= r - 1ull;
 			const uint64_t m = triMask[b];
 			if ((m >> lane) & 1ull)
 			{
-				const SrpTri& T = recs[b];
+				const SrpTri& T = recs[(head + b) & 63u];
 				float l[3];
-				baryAt(T, px, py, l);
+				baryFromBase(T, base, j, px, py, tx, l);
 				const uint32_t pos = fragBase[b] + (uint32_t) __popcll(m & ltMask);
 				fragL[0][pos] = l[0];
 				fragL[1][pos] = l[1];
@@ -644,6 +727,8 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				fragPix[pos] = (uint8_t) lane;
 				fragTri[pos] = (uint8_t) b;
 			}
+			}
+			__syncthreads();
 		}
 		const uint64_t knownLanes = __ballot(known);
 		__syncthreads();
@@ -666,7 +751,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			if (f < nFrag)
 			{
 				const uint32_t b = fragTri[f];
-				const SrpTri& T = recs[b];
+				const SrpTri& T = recs[(head + b) & 63u];
 				const int fx = (int) (tx * 8 + (p & 7u)), fy = (int) (ty * 8 + (p >> 3));
 				float l[3];
 				if ((knownLanes >> b) & 1ull)
@@ -759,7 +844,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			const uint8_t state = fragState[f];
 			if (state == FRAG_SCISSORED)
 				continue;
-			const bool front = (recs[b].flags >> 3) & 1u;
+			const bool front = (recs[(head + b) & 63u].flags >> 3) & 1u;
 			const SRPStencilFaceState& sf = st.stencil[front ? 0 : 1];
 			if (st.stencilEnabled && !compare<uint32_t>(sf.func, (uint32_t) (sf.ref & sf.mask), sten & sf.mask))
 			{
@@ -783,6 +868,8 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		}
 		__syncthreads();
 		i += nb;
+		head = (head + nb) & 63u;
+		filled -= nb;
 	}
 	if (inFb)
 	{

@@ -4,7 +4,12 @@ ref` from /root/reference).  For each case: SHA-256 of the colour, depth and
 stencil buffers the reference produces.  These pin the oracle (CPU tests)
 and the GPU path (GPU tests) without the reference itself being present.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py            # reference_hashes.json
+    python tests/golden/make_fixtures.py --heavy    # reference_hashes_heavy.json
+
+The heavy cases are the full-size workloads whose CPU frames take minutes
+(the primitive mix at 1920x1080 covers 3.9e9 fragments); the GPU tests
+compare their frames with these hashes directly.
 """
 import hashlib
 import json
@@ -32,6 +37,14 @@ def cases(tex):
     return out
 
 
+def heavy_cases():
+    return [
+        dict(name="cfg/mix", width=1920, height=1080, p0=500000, p1=100000, p2=100000, seed=0x5EED0003),
+        dict(name="cfg/terrain", width=1920, height=1080, p2=0),
+        dict(name="cfg/terrain", width=1920, height=1080, p2=300),
+    ]
+
+
 def case_id(c):
     return ",".join(f"{k}={v}" for k, v in sorted(c.items()) if k != "tex_path")
 
@@ -47,13 +60,24 @@ def digest(frame):
 
 def main():
     ref = SceneModule(REF_SO)
-    tex = write_noise_png("/tmp/srp_fixture_noise1024.png", 1024, NOISE_SEED)
+    heavy = "--heavy" in sys.argv
     table = {}
-    for c in cases(tex):
-        table[case_id(c)] = digest(ref.run(**c))
-    with open(os.path.join(HERE, "reference_hashes.json"), "w") as f:
-        json.dump({"noise_seed": NOISE_SEED, "cases": table}, f, indent=1, sort_keys=True)
-    print(f"wrote {len(table)} cases")
+    if heavy:
+        for c in heavy_cases():
+            ref.reset_stats("srpRefResetStats") if hasattr(ref.lib, "srpRefResetStats") else None
+            table[case_id(c)] = digest(ref.run(**c))
+            print("done", case_id(c), flush=True)
+        out = "reference_hashes_heavy.json"
+        meta = {}
+    else:
+        tex = write_noise_png("/tmp/srp_fixture_noise1024.png", 1024, NOISE_SEED)
+        for c in cases(tex):
+            table[case_id(c)] = digest(ref.run(**c))
+        out = "reference_hashes.json"
+        meta = {"noise_seed": NOISE_SEED}
+    with open(os.path.join(HERE, out), "w") as f:
+        json.dump(dict(meta, cases=table), f, indent=1, sort_keys=True)
+    print(f"wrote {len(table)} cases to {out}")
 
 
 if __name__ == "__main__":
