@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 7u
+#define SRP_ABI_VERSION 8u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -168,6 +168,8 @@ typedef struct SrpRasterLaunch
 	uint32_t tile;              /* tile side: 8 or 16 */
 	uint32_t nBlocks;           /* grid size (for the XCD-aware tile order) */
 	uint32_t batched;           /* batched three-phase kernel (tile 8, one wave per tile) */
+	uint32_t bigPixel;          /* batched: triangles without exact masks are shaded
+	                               pixel-parallel in runs instead of via fragment lists */
 	/* multi-draw persistent mode (nSeg > 0): nTiles counts the tiles of all
 	 * draws, seg[d] covers draw d's tiles [first, first + count), and the
 	 * nBlocks waves take tiles from `queues` (8 counters, one per range of

@@ -639,7 +639,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		__syncthreads();
 		/* Triangles without an exact mask: pixel-parallel coverage, 8
 		 * triangles per round from shared row bases (bits are slots). */
-		uint64_t unknown = __ballot(fresh && !sknown);
+		uint64_t unknown = L.bigPixel ? 0ull : __ballot(fresh && !sknown);
 		while (unknown)
 		{
 			const uint64_t round = takeRound(unknown);
@@ -671,15 +671,139 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		const uint32_t mhi = (uint32_t) __shfl((uint32_t) (smask >> 32), (int) src);
 		const uint64_t mask = ((uint64_t) mhi << 32) | mlo;
 		const bool known = __shfl((int) sknown, (int) src) != 0;
+		uint32_t nb;
+		const uint64_t unkWin = __ballot(lane < avail && !known);
+		if (L.bigPixel && (unkWin & 1ull))
+		{
+			/* A run of up to 8 triangles without exact masks heads the
+			 * window: lane = pixel, the triangles in order, each fragment
+			 * tested and shaded as the reference does, with no fragment
+			 * list (these triangles cover much of the tile). */
+			const uint64_t notUnk = ~unkWin;
+			uint32_t run = notUnk ? (uint32_t) __builtin_ctzll(notUnk) : 64u;
+			if (run > 8u)
+				run = 8u;
+			rowBases(recs, head, (1ull << run) - 1ull, tx, ty, lane, base);
+			__syncthreads();
+			for (uint32_t j = 0; j < run; j++)
+			{
+				const SrpTri& T = recs[(head + j) & 63u];
+				bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
+				float l[3] = {0.0f, 0.0f, 0.0f};
+				if (cov)
+				{
+					baryFromBase(T, base, j, px, py, tx, l);
+					cov = insideTri(T, l);
+				}
+				const bool front = (T.flags >> 3) & 1u;
+				const SRPStencilFaceState& sf = st.stencil[front ? 0 : 1];
+				const float iw[3] = {T.invW[0], T.invW[1], T.invW[2]};
+				float depth = 0.0f, rec = 0.0f;
+				bool active = false;
+				if (cov)
+				{
+					covered++;
+					rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
+					depth = T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2];
+					active = true;
+					if (st.scissorEnabled)
+					{
+						const uint64_t ux = (uint64_t) (int64_t) px, uy = (uint64_t) (int64_t) py;
+						if (ux < st.sx0 || ux >= st.sx1 || uy < st.sy0 || uy >= st.sy1)
+							active = false;
+					}
+					if (active && st.stencilEnabled &&
+					    !compare<uint32_t>(sf.func, (uint32_t) (sf.ref & sf.mask), sten & sf.mask))
+					{
+						sten = stencilApply(sf.sfailOp, sten, sf);
+						active = false;
+					}
+					if (active && !LATE && st.depthTest && !compare<float>(st.depthOp, depth, dep))
+					{
+						if (st.stencilEnabled) sten = stencilApply(sf.dfailOp, sten, sf);
+						active = false;
+					}
+				}
+				SRPFragmentShaderIn in;
+				SRPFragmentShaderOut out;
+				out.color[0] = 0.0f; out.color[1] = 0.0f; out.color[2] = 0.0f; out.color[3] = 0.0f;
+				out.fragDepth = __builtin_nanf("");
+				typedef typename VaryingBuf<WORDS>::type BufT;
+				BufT buf[VaryingBuf<WORDS>::n];
+				in.varyings = (SRPInterpolated*) buf;
+				in.uniform = (SRPUniform*) P->uniform;
+				in.fragCoord[0] = (float) ((double) px + 0.5);
+				in.fragCoord[1] = (float) ((double) py + 0.5);
+				in.fragCoord[2] = depth;
+				in.fragCoord[3] = rec;
+				in.frontFacing = front;
+				in.primitiveID = T.id;
+				if (active)
+				{
+					if (WORDS)
+					{
+						const uint32_t* v[3];
+#pragma unroll
+						for (int k = 0; k < 3; k++) v[k] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T.vref[k]));
+						interpWords<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint32_t*>(buf));
+					}
+					else
+					{
+						const uint8_t* v[3];
+#pragma unroll
+						for (int k = 0; k < 3; k++) v[k] = varyingPtr(P, T.vref[k]);
+						uint8_t* bb = reinterpret_cast<uint8_t*>(buf);
+						for (uint32_t q = 0; q < P->describedSize; q++) bb[q] = 0;
+						interpBytes<3>(P, v, l, iw, rec, pv, bb);
+					}
+				}
+				if constexpr (FsIsWave<FS>::value)
+				{
+					if (!active)
+						for (uint32_t q = 0; q < VaryingBuf<WORDS>::n; q++) buf[q] = 0;
+					FS().wave(&in, &out, active);
+				}
+				else if (active)
+					FS()(&in, &out);
+				if (active)
+				{
+					shaded++;
+					float d = depth;
+					bool ok = true;
+					if (LATE)
+					{
+						if (!isnan(out.fragDepth)) d = out.fragDepth;
+						if (st.depthTest && !compare<float>(st.depthOp, d, dep))
+						{
+							if (st.stencilEnabled) sten = stencilApply(sf.dfailOp, sten, sf);
+							ok = false;
+						}
+					}
+					if (ok)
+					{
+						if (st.stencilEnabled) sten = stencilApply(sf.passOp, sten, sf);
+						col = packColor(out.color);
+						if (st.depthTest && st.depthWrite)
+							dep = d;
+					}
+				}
+			}
+			__syncthreads();
+			nb = run;
+		}
+		else
+		{
+		/* entries taken by the fragment-list path this time */
+		const uint32_t lim = (L.bigPixel && unkWin) ? (uint32_t) __builtin_ctzll(unkWin) : avail;
 		/* Inclusive scan of fragment counts; keep the longest prefix that fits. */
-		const uint32_t cnt = (lane < avail) ? (uint32_t) __popcll(mask) : 0u;
+		const uint32_t cnt = (lane < lim) ? (uint32_t) __popcll(mask) : 0u;
 		uint32_t incl = cnt;
 		for (int off = 1; off < 64; off <<= 1)
 		{
 			const uint32_t v = __shfl_up(incl, off);
 			if ((int) lane >= off) incl += v;
 		}
-		const uint32_t nb = (uint32_t) __popcll(__ballot(lane < avail && incl <= (uint32_t) RF));
+		nb = (uint32_t) __popcll(__ballot(lane < lim && incl <= (uint32_t) RF));
 		const uint32_t nFrag = __shfl(incl, (int) nb - 1);
 		if (lane < nb)
 		{
@@ -867,6 +991,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				dep = d;
 		}
 		__syncthreads();
+		}
 		i += nb;
 		head = (head + nb) & 63u;
 		filled -= nb;

@@ -68,6 +68,13 @@ void srpHipProfileRead(double ms[4], uint64_t launches[4]);
 /** Raster tuning: tile side in pixels (8 or 16). */
 void srpHipSetTileSize(int tile);
 int srpHipGetTileSize(void);
+/** Raster variant of single draws (none changes the result): batched = 1:
+ *  the three-phase wave-per-tile kernel (tile 8), 0: one lane per pixel;
+ *  pixelRuns = 1: in the batched kernel, triangles without exact binning
+ *  masks are shaded pixel-parallel in runs, 0: through fragment lists;
+ *  maskBox: exact 8x8 binning masks for triangle boxes up to this side
+ *  (0..24).  A negative argument leaves that setting unchanged. */
+void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox);
 
 /** Shader registration (see srp/srp_device.h).  `entry` points to a
  *  struct SrpVsEntry / SrpFsEntry of srp/device/abi.h. */
@@ -103,6 +110,7 @@ typedef struct SRPBatchTuning
 	int wavesPerCU;   /**< persistent rasteriser waves per compute unit (1..32) */
 	int maskBox;      /**< exact 8x8 binning masks for triangle boxes up to this side (0..24 px) */
 	int xcdQueues;    /**< 1: one tile queue per XCD range (work stealing between them); 0: one queue */
+	int pixelRuns;    /**< 1: triangles without exact masks are shaded pixel-parallel in runs; 0: via fragment lists */
 } SRPBatchTuning;
 
 typedef struct SRPBatch SRPBatch;
@@ -121,7 +129,9 @@ int srpBatchCapture(SRPBatch* batch);
 /** Change the tuning (drops a captured graph) / read it. */
 void srpBatchSetTuning(SRPBatch* batch, const SRPBatchTuning* tuning);
 void srpBatchGetTuning(const SRPBatch* batch, SRPBatchTuning* tuning);
-/** Time `trials` frames of every candidate tuning and keep the fastest;
+/** Time `trials` frames of every candidate tuning (36: waves per CU 4, 8,
+ *  16 x mask box 0, 16, 24 x both queue modes x both run modes) and keep
+ *  the fastest;
  *  returns its mean frame time in ms and writes it to *best (if not NULL). */
 double srpBatchAutotune(SRPBatch* batch, int trials, SRPBatchTuning* best);
 /** draws, assembled primitives, binned (tile, triangle) pairs, tiles of the last frame sizing */

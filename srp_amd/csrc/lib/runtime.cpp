@@ -112,6 +112,7 @@ struct Runtime
 	int tile = 8;
 	int batched = 1;  /* three-phase batched rasteriser (tile 8) */
 	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
+	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
 
 	DevBuf uniform, clip, varyings, genVaryings, counts, totals;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount;
@@ -169,6 +170,7 @@ struct Runtime
 		if (t && (atoi(t) == 8 || atoi(t) == 16)) tile = atoi(t);
 		const char* b = getenv("SRP_RASTER");
 		if (b && strcmp(b, "simple") == 0) batched = 0;
+		if (b && strcmp(b, "lists") == 0) bigPixel = 0;
 	}
 };
 
@@ -843,6 +845,14 @@ void srpHipSetTileSize(int t)
 }
 int srpHipGetTileSize(void) { return rt().tile; }
 
+void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox)
+{
+	Runtime& R = rt();
+	if (batched >= 0) R.batched = batched ? 1 : 0;
+	if (pixelRuns >= 0) R.bigPixel = pixelRuns ? 1u : 0u;
+	if (maskBox >= 0) R.maskBox = (uint32_t) (maskBox > SRP_SMALL_BBOX ? SRP_SMALL_BBOX : maskBox);
+}
+
 }  // extern "C"
 
 /* =================================================================== draw */
@@ -1216,6 +1226,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		rl.words = H.wordPath;
 		rl.tile = (uint32_t) R.tile;
 		rl.batched = (R.batched && R.tile == 8) ? 1u : 0u;
+		rl.bigPixel = R.bigPixel;
 		const uint32_t perBlock = rl.batched ? 1u : ((R.tile == 8) ? 4u : 1u);
 		rl.nBlocks = (nTiles + perBlock - 1) / perBlock;
 		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
@@ -1315,7 +1326,7 @@ struct SRPBatch
 	uint32_t* queues = nullptr;
 	ScanScratch scan;
 	SortScratch sort;
-	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1};
+	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1, 1};
 	int cus = 256;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -1460,6 +1471,7 @@ void batchLaunch(SRPBatch* B, hipStream_t s)
 	rl.words = B->words;
 	rl.tile = 8;
 	rl.batched = 1;
+	rl.bigPixel = B->tuning.pixelRuns ? 1u : 0u;
 	uint32_t waves = (uint32_t) (B->cus * B->tuning.wavesPerCU);
 	if (waves > B->totalTiles) waves = B->totalTiles;
 	rl.nBlocks = waves ? waves : 1u;
@@ -1766,9 +1778,10 @@ double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 		return 0.0;
 	if (trials < 1) trials = 1;
 	Runtime& R = rt();
-	const int waves[] = {2, 4, 8, 16};
-	const int boxes[] = {0, 8, 16, 24};
+	const int waves[] = {4, 8, 16};
+	const int boxes[] = {0, 16, 24};
 	const int queues[] = {0, 1};
+	const int runs[] = {0, 1};
 	SRPBatchTuning bestT = B->tuning;
 	double bestMs = 1e30;
 	hipEvent_t a, b;
@@ -1777,8 +1790,9 @@ double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 	for (int box : boxes)
 		for (int w : waves)
 			for (int q : queues)
+				for (int pr : runs)
 			{
-				SRPBatchTuning t = {w, box, q};
+				SRPBatchTuning t = {w, box, q, pr};
 				srpBatchSetTuning(B, &t);
 				batchFlushPending(B);
 				batchLaunch(B, R.stream);   /* warm-up */

@@ -281,19 +281,20 @@ void srpBenchBatchRead(int k, uint32_t* color, float* depth, uint8_t* stencil)
 #ifdef SRP_GPU_BUILD
 int srpBenchBatchCapture(void) { return g_bb.batch ? srpBatchCapture(g_bb.batch) : -1; }
 
-double srpBenchBatchAutotune(int trials, int out[3])
+double srpBenchBatchAutotune(int trials, int out[4])
 {
 	SRPBatchTuning t;
 	const double ms = srpBatchAutotune(g_bb.batch, trials, &t);
 	out[0] = t.wavesPerCU;
 	out[1] = t.maskBox;
 	out[2] = t.xcdQueues;
+	out[3] = t.pixelRuns;
 	return ms;
 }
 
-void srpBenchBatchSetTuning(int waves, int box, int xcd)
+void srpBenchBatchSetTuning(int waves, int box, int xcd, int runs)
 {
-	SRPBatchTuning t = {waves, box, xcd};
+	SRPBatchTuning t = {waves, box, xcd, runs};
 	srpBatchSetTuning(g_bb.batch, &t);
 }
 

@@ -71,10 +71,11 @@ def test_tunings_give_identical_frames(gpu_batch):
     try:
         gb.frame(True)
         base = gb.read_all()
-        for waves, box, xcd in [(1, 0, 0), (2, 8, 1), (16, 16, 0), (32, 24, 1), (4, 0, 1)]:
-            gb.set_tuning(waves, box, xcd)
+        for waves, box, xcd, runs in [(1, 0, 0, 0), (2, 8, 1, 1), (16, 16, 0, 0), (32, 24, 1, 1), (4, 0, 1, 1),
+                                      (8, 24, 0, 0)]:
+            gb.set_tuning(waves, box, xcd, runs)
             gb.frame(True)
-            assert frames_equal(gb.read_all(), base) == [], (waves, box, xcd)
+            assert frames_equal(gb.read_all(), base) == [], (waves, box, xcd, runs)
     finally:
         gb.finish()
 
@@ -85,10 +86,11 @@ def test_autotuner_picks_a_candidate_and_keeps_results(gpu_batch):
     try:
         gb.frame(True)
         base = gb.read_all()
-        ms, (waves, box, xcd) = gb.autotune(2)
-        print(f"[batch] autotune: {ms:.3f} ms with waves/CU {waves}, mask box {box}, xcd queues {xcd}", flush=True)
+        ms, (waves, box, xcd, runs) = gb.autotune(2)
+        print(f"[batch] autotune: {ms:.3f} ms with waves/CU {waves}, mask box {box}, xcd queues {xcd}, "
+              f"pixel runs {runs}", flush=True)
         assert ms > 0
-        assert waves in (2, 4, 8, 16) and box in (0, 8, 16, 24) and xcd in (0, 1)
+        assert waves in (4, 8, 16) and box in (0, 16, 24) and xcd in (0, 1) and runs in (0, 1)
         gb.frame(True)
         assert frames_equal(gb.read_all(), base) == []
     finally:

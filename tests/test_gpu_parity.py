@@ -91,6 +91,26 @@ def test_cfg_terrain_full_grid(oracle, gpu, pose):
     assert_same(a, b, sa, sb, f"cfg/terrain pose {pose}")
 
 
+@pytest.mark.parametrize("batched,runs,box", [(1, 0, 24), (1, 1, 0), (1, 0, 0), (0, 1, 24), (1, 1, 8)])
+def test_raster_variants(oracle, gpu, batched, runs, box):
+    """Every single-draw raster variant gives the oracle's frames."""
+    import ctypes
+    lib = gpu.lib
+    lib.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
+    lib.srpHipSetRasterOptions(batched, runs, box)
+    try:
+        cases = [dict(name="cfg/tris", p0=2000, seed=0x5EED0001),
+                 dict(name="cfg/mix", width=480, height=270, p0=20000, p1=4000, p2=4000, seed=3),
+                 dict(name="cfg/terrain", width=480, height=270, p0=200, p1=100, p2=7),
+                 dict(name="misc/stencil_test"), dict(name="interpolation/perspective")]
+        cases += [dict(name="cfg/fuzz", seed=s) for s in range(0, 40, 3)]
+        for c in cases:
+            a, b, sa, sb = run_both(oracle, gpu, **c)
+            assert_same(a, b, sa, sb, f"{c} variant {(batched, runs, box)}")
+    finally:
+        lib.srpHipSetRasterOptions(1, 1, 24)
+
+
 def test_cfg_mesh_small(oracle, gpu, noise_tex):
     a, b, sa, sb = run_both(oracle, gpu, "cfg/mesh", width=480, height=270, p0=32, p1=64, tex_path=noise_tex)
     assert_same(a, b, sa, sb, "cfg/mesh small")
