@@ -11,10 +11,16 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 8u
+#define SRP_ABI_VERSION 9u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
+/* Fragment counters (SrpDrawParams.stats): SRP_STAT_SLOTS slots of
+ * SRP_STAT_STRIDE u64 (one 128-byte line each), slot s holding covered at
+ * [s * STRIDE] and shaded at [s * STRIDE + 1]; waves add to slot
+ * (block % SLOTS) so that their atomics do not all meet on one address. */
+#define SRP_STAT_SLOTS 64
+#define SRP_STAT_STRIDE 16
 
 /* interpolation kinds of a 4-byte word in the fast path */
 enum SrpWordKind
@@ -127,7 +133,7 @@ typedef struct SrpDrawParams
 	uint32_t maskBox;        /* triangles whose box is at most this many pixels
 	                            on each side get exact 8x8 masks at binning */
 
-	unsigned long long* stats;   /* covered, shaded, primitives, draws */
+	unsigned long long* stats;   /* fragment counters, SRP_STAT_SLOTS slots */
 } SrpDrawParams;
 
 /* One draw of a multi-draw launch: its parameters and its range of the

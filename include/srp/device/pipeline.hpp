@@ -298,8 +298,9 @@ __device__ __forceinline__ void flushStats(const SrpDrawParams* P, uint32_t cove
 	}
 	if ((threadIdx.x & 63) == 0 && (covered | shaded))
 	{
-		atomicAdd(&P->stats[0], (unsigned long long) covered);
-		atomicAdd(&P->stats[1], (unsigned long long) shaded);
+		unsigned long long* c = P->stats + (size_t) (blockIdx.x % SRP_STAT_SLOTS) * SRP_STAT_STRIDE;
+		atomicAdd(&c[0], (unsigned long long) covered);
+		atomicAdd(&c[1], (unsigned long long) shaded);
 	}
 }
 
@@ -528,8 +529,8 @@ __device__ __forceinline__ bool insideTri(const SrpTri& T, const float* l)
  * first column inside the triangle's box, as the reference's scan would
  * reach them; a pixel of the row then continues with at most 7 additions.
  * (A walk split in two is the same sequence of additions.) */
-__device__ __forceinline__ void rowBases(const SrpTri* recs, uint32_t head, uint64_t round, uint32_t tx, uint32_t ty,
-                                         uint32_t lane, float (*base)[8][3])
+__device__ __forceinline__ void rowBases(const SrpTri* tris, const uint32_t* recIdx, uint32_t head, uint64_t round,
+                                         uint32_t tx, uint32_t ty, uint32_t lane, float (*base)[8][3])
 {
 	const uint32_t j = lane & 7u, i = lane >> 3;
 	uint64_t r = round;
@@ -537,7 +538,7 @@ __device__ __forceinline__ void rowBases(const SrpTri* recs, uint32_t head, uint
 		r &= r - 1ull;
 	if (!r)
 		return;
-	const SrpTri& T = recs[(head + (uint32_t) __builtin_ctzll(r)) & 63u];
+	const SrpTri& T = tris[recIdx[(head + (uint32_t) __builtin_ctzll(r)) & 63u]];
 	const int y = (int) (ty * 8u + i);
 	if (y < (int) T.y0 || y >= (int) T.y1)
 		return;
@@ -578,9 +579,9 @@ __device__ __forceinline__ uint64_t takeRound(uint64_t& rest)
 
 template <class FS, bool LATE, bool WORDS>
 __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, const SrpDrawParams* P, uint32_t tile,
-                                                  uint32_t gtile)
+                                                  uint32_t gtile, uint32_t& covered, uint32_t& shaded)
 {
-	__shared__ SrpTri recs[64];
+	__shared__ uint32_t recIdx[64];   /* record of each window slot (records stay in L2) */
 	__shared__ float base[8][8][3];
 	__shared__ unsigned long long triMask[64];
 	__shared__ uint32_t fragBase[64];
@@ -603,7 +604,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
 	const SrpState& st = P->st;
 
-	uint32_t col = 0, sten = 0, covered = 0, shaded = 0;
+	uint32_t col = 0, sten = 0;
 	float dep = 0.0f;
 	if (inFb)
 	{
@@ -615,7 +616,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	const bool precull = !LATE && st.depthTest && monotoneOp(st.depthOp);
 
 	/* The tile's list is consumed through a window of 64 slots: slot s
-	 * (lane s keeps its mask, recs[s] its record) holds list entry
+	 * (lane s keeps its mask, recIdx[s] its record) holds list entry
 	 * i + ((s - head) mod 64).  Each entry is loaded and its mask found once,
 	 * however many batches it waits before its fragments fit. */
 	uint32_t head = 0, filled = 0;
@@ -634,7 +635,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			sknown = (pr >> 31) != 0;
 			smask = sknown ? L.pairMask[pi] : 0ull;
 			if (pr != 0xFFFFFFFFu)   /* a pair without a record (capacity exceeded) covers nothing */
-				recs[lane] = L.tris[pr & 0x7FFFFFFFu];
+				recIdx[lane] = pr & 0x7FFFFFFFu;
 		}
 		__syncthreads();
 		/* Triangles without an exact mask: pixel-parallel coverage, 8
@@ -643,14 +644,14 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		while (unknown)
 		{
 			const uint64_t round = takeRound(unknown);
-			rowBases(recs, 0u, round, tx, ty, lane, base);
+			rowBases(L.tris, recIdx, 0u, round, tx, ty, lane, base);
 			__syncthreads();
 			uint64_t r = round;
 			for (uint32_t j = 0; r; j++)
 			{
 				const uint32_t b = (uint32_t) __builtin_ctzll(r);
 				r &= r - 1ull;
-				const SrpTri& T = recs[b];
+				const SrpTri& T = L.tris[recIdx[b]];
 				bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
 				if (cov)
 				{
@@ -683,11 +684,11 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			uint32_t run = notUnk ? (uint32_t) __builtin_ctzll(notUnk) : 64u;
 			if (run > 8u)
 				run = 8u;
-			rowBases(recs, head, (1ull << run) - 1ull, tx, ty, lane, base);
+			rowBases(L.tris, recIdx, head, (1ull << run) - 1ull, tx, ty, lane, base);
 			__syncthreads();
 			for (uint32_t j = 0; j < run; j++)
 			{
-				const SrpTri& T = recs[(head + j) & 63u];
+				const SrpTri& T = L.tris[recIdx[(head + j) & 63u]];
 				bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
 				float l[3] = {0.0f, 0.0f, 0.0f};
 				if (cov)
@@ -831,7 +832,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		while (unk)
 		{
 			const uint64_t round = takeRound(unk);
-			rowBases(recs, head, round, tx, ty, lane, base);
+			rowBases(L.tris, recIdx, head, round, tx, ty, lane, base);
 			__syncthreads();
 			uint64_t r = round;
 			for (uint32_t j = 0; r; j++)
@@ -841,7 +842,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			const uint64_t m = triMask[b];
 			if ((m >> lane) & 1ull)
 			{
-				const SrpTri& T = recs[(head + b) & 63u];
+				const SrpTri& T = L.tris[recIdx[(head + b) & 63u]];
 				float l[3];
 				baryFromBase(T, base, j, px, py, tx, l);
 				const uint32_t pos = fragBase[b] + (uint32_t) __popcll(m & ltMask);
@@ -875,7 +876,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			if (f < nFrag)
 			{
 				const uint32_t b = fragTri[f];
-				const SrpTri& T = recs[(head + b) & 63u];
+				const SrpTri& T = L.tris[recIdx[(head + b) & 63u]];
 				const int fx = (int) (tx * 8 + (p & 7u)), fy = (int) (ty * 8 + (p >> 3));
 				float l[3];
 				if ((knownLanes >> b) & 1ull)
@@ -968,7 +969,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			const uint8_t state = fragState[f];
 			if (state == FRAG_SCISSORED)
 				continue;
-			const bool front = (recs[(head + b) & 63u].flags >> 3) & 1u;
+			const bool front = (L.tris[recIdx[(head + b) & 63u]].flags >> 3) & 1u;
 			const SRPStencilFaceState& sf = st.stencil[front ? 0 : 1];
 			if (st.stencilEnabled && !compare<uint32_t>(sf.func, (uint32_t) (sf.ref & sf.mask), sten & sf.mask))
 			{
@@ -1002,7 +1003,6 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		P->depth[idx] = dep;
 		P->stencil[idx] = (uint8_t) sten;
 	}
-	flushStats(P, covered, shaded);
 }
 
 template <class FS, bool LATE, bool WORDS>
@@ -1011,7 +1011,9 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
 	if (tile >= L.nTiles)
 		return;
-	rasterTileBatched<FS, LATE, WORDS>(L, L.params, tile, tile);
+	uint32_t covered = 0, shaded = 0;
+	rasterTileBatched<FS, LATE, WORDS>(L, L.params, tile, tile, covered, shaded);
+	flushStats(L.params, covered, shaded);
 }
 
 /* Persistent multi-draw rasteriser: L.nBlocks waves take tiles of all the
@@ -1025,6 +1027,8 @@ __global__ __launch_bounds__(64) void srp_raster_persistent(SrpRasterLaunch L)
 {
 	const uint32_t nq = L.xcdQueues ? 8u : 1u;
 	const uint32_t q0 = blockIdx.x % nq;
+	uint32_t covered = 0, shaded = 0;
+	const SrpDrawParams* last = nullptr;
 	for (uint32_t k = 0; k < nq; k++)
 	{
 		const uint32_t q = (q0 + k) % nq;
@@ -1047,9 +1051,13 @@ __global__ __launch_bounds__(64) void srp_raster_persistent(SrpRasterLaunch L)
 				else b = m;
 			}
 			const SrpSeg sg = L.seg[a];
-			rasterTileBatched<FS, LATE, WORDS>(L, sg.P, t - sg.first, t);
+			rasterTileBatched<FS, LATE, WORDS>(L, sg.P, t - sg.first, t, covered, shaded);
+			last = sg.P;
 		}
 	}
+	/* every draw of a batch shares one counter array */
+	if (last)
+		flushStats(last, covered, shaded);
 }
 
 /* Lines and points: fragments sorted (stably) by pixel; the first thread

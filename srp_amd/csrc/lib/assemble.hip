@@ -487,7 +487,7 @@ __device__ void emitLineFrags(const AsmArgs& A, const SrpDrawParams* P, const Sr
 		x += xInc; y += yInc; t += tInc;
 	}
 	if (invalid)
-		atomicAdd(&P->stats[0], (unsigned long long) invalid);
+		atomicAdd(&P->stats[(size_t) (blockIdx.x % SRP_STAT_SLOTS) * SRP_STAT_STRIDE], (unsigned long long) invalid);
 }
 
 /* point.c:81-118: pixel bounds of a point; false when fully outside. */

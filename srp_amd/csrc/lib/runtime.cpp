@@ -97,6 +97,7 @@ struct DevBuf
 constexpr size_t kAsmDrawOff = (sizeof(SrpDrawParams) + 63) & ~(size_t) 63;
 constexpr size_t kParamsHead = (kAsmDrawOff + sizeof(AsmDraw) + 255) & ~(size_t) 255;
 constexpr size_t kMaxInlineUniform = 16384;
+constexpr size_t kStatWords = (size_t) SRP_STAT_SLOTS * SRP_STAT_STRIDE;
 constexpr size_t kParamsBlock = kParamsHead + kMaxInlineUniform;
 
 struct Runtime
@@ -163,8 +164,8 @@ struct Runtime
 		HIP_CHECK(hipHostMalloc((void**) &hParams, kParamsBlock, hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dParams, kParamsBlock));
 		HIP_CHECK(hipHostMalloc((void**) &hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
-		HIP_CHECK(hipMalloc((void**) &dStats, 4 * sizeof(unsigned long long)));
-		HIP_CHECK(hipMemset(dStats, 0, 4 * sizeof(unsigned long long)));
+		HIP_CHECK(hipMalloc((void**) &dStats, kStatWords * sizeof(unsigned long long)));
+		HIP_CHECK(hipMemset(dStats, 0, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
 		const char* t = getenv("SRP_TILE");
 		if (t && (atoi(t) == 8 || atoi(t) == 16)) tile = atoi(t);
@@ -803,11 +804,15 @@ void srpHipShaderCalledOnHost(const char* name)
 void srpHipGetStats(uint64_t out[4])
 {
 	Runtime& R = rt();
-	unsigned long long d[4];
+	std::vector<unsigned long long> d(kStatWords);
 	HIP_CHECK(hipStreamSynchronize(R.stream));
-	HIP_CHECK(hipMemcpy(d, R.dStats, sizeof(d), hipMemcpyDeviceToHost));
-	out[0] = d[0];
-	out[1] = d[1];
+	HIP_CHECK(hipMemcpy(d.data(), R.dStats, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+	out[0] = out[1] = 0;
+	for (size_t s = 0; s < SRP_STAT_SLOTS; s++)
+	{
+		out[0] += d[s * SRP_STAT_STRIDE];
+		out[1] += d[s * SRP_STAT_STRIDE + 1];
+	}
 	out[2] = R.hostPrims;
 	out[3] = R.hostDraws;
 }
@@ -816,7 +821,7 @@ void srpHipResetStats(void)
 {
 	Runtime& R = rt();
 	HIP_CHECK(hipStreamSynchronize(R.stream));
-	HIP_CHECK(hipMemset(R.dStats, 0, 4 * sizeof(unsigned long long)));
+	HIP_CHECK(hipMemset(R.dStats, 0, kStatWords * sizeof(unsigned long long)));
 	R.hostPrims = 0;
 	R.hostDraws = 0;
 }

@@ -53,6 +53,8 @@ typedef struct Bench
 
 static Bench g_b;
 
+void srpBenchFrame(void);
+
 static void benchFbInit(int w, int h)
 {
 	g_b.width = w;
