@@ -128,7 +128,10 @@ def neural_leg(args):
         raise RuntimeError("terrain setup failed")
     ds = nn.new_ds(8_000_000)
     net = nn.new_net(0x5EED0005)
-    out = {}
+    if args.nn_freq:
+        st = [float(v) for v in args.nn_freq.split(",")]
+        nn.set_encoding(net, st * 8 if len(st) == 1 else st)
+    out = {"freq_steps": [round(float(v), 4) for v in nn.get_encoding(net)]}
     try:
         t0 = time.perf_counter()
         got = nn.terrain_collect(ds, 0, 256, args.nn_keep, 0x5EED0005)
@@ -327,6 +330,10 @@ def main():
     ap.add_argument("--nn-batch", type=int, default=8192)
     ap.add_argument("--nn-lr", type=float, default=5e-3)
     ap.add_argument("--nn-frames", type=int, default=50)
+    ap.add_argument("--nn-freq", default="1.0,0.5,0.5,0.5,0.5,0.5,2.0,1.0",
+                    help="encoding frequency step: one value, or 8 comma-separated (per feature: position xyz, "
+                         "normal xyz, uv); the default doubles x and quadruples u over the library's 0.5, which "
+                         "the terrain's fBm albedo (period ~1/12 of the x extent) needs")
     ap.add_argument("--neural-only", action="store_true", help="run only the neural-shader leg and print its JSON")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-shader leg")
     ap.add_argument("--batch-scenes", type=int, default=64)

@@ -56,13 +56,14 @@ __device__ __forceinline__ void sincosRev(float r, float& s, float& c)
 	c = __builtin_amdgcn_cosf(r);
 }
 
-/* Encoding values of feature value x at frequencies j0..j0+3, interleaved
- * sin, cos; (sd, cd) = sin, cos of one frequency step at x.  The first pair
- * is evaluated directly, the next three by rotation. */
-__device__ __forceinline__ void encode4(float x, int j0, float sd, float cd, float out[8])
+/* Encoding values of a feature at frequencies j0..j0+3, interleaved sin,
+ * cos; u = the feature times its frequency step, (sd, cd) = sin, cos of
+ * 2*pi*u.  The first pair is evaluated directly, the next three by
+ * rotation. */
+__device__ __forceinline__ void encode4(float u, int j0, float sd, float cd, float out[8])
 {
 	float s, c;
-	sincosRev(SRP_NN_FREQ_STEP * (float) (j0 + 1) * x, s, c);
+	sincosRev((float) (j0 + 1) * u, s, c);
 	out[0] = s;
 	out[1] = c;
 #pragma unroll
@@ -77,10 +78,10 @@ __device__ __forceinline__ void encode4(float x, int j0, float sd, float cd, flo
 	}
 }
 
-__device__ __forceinline__ bf16x8 encode8(float x, int j0, float sd, float cd)
+__device__ __forceinline__ bf16x8 encode8(float u, int j0, float sd, float cd)
 {
 	float v[8];
-	encode4(x, j0, sd, cd, v);
+	encode4(u, j0, sd, cd, v);
 	bf16x8 o;
 #pragma unroll
 	for (int q = 0; q < 8; q++)
@@ -139,22 +140,35 @@ __device__ __forceinline__ void packOperands(const f32x4 acc[4][4], bf16x8 h[2][
 }
 
 /* Layer 1 for the wave's 64 fragments: acc[mb][nb] = W1 X^T (no bias).
- * x[nb][d] = feature d of fragment 16nb + c.  If xt != nullptr the encoded
- * operand is also stored transposed (xt[k * ld + col0 + 16nb + c]). */
-__device__ __forceinline__ void layer1(const uint16_t* __restrict__ w1, const float x[4][8], int g, int c,
-                                       f32x4 acc[4][4], __bf16* xt = nullptr, size_t ld = 0)
+ * xin = this lane's own fragment's features, step = the network's
+ * frequency steps; column 16nb + c of X^T is lane 16nb + c's fragment,
+ * fetched by shuffle one feature at a time (so no lane holds all four
+ * columns' features at once).  If xt != nullptr the encoded operand is also
+ * stored transposed (xt[k * ld + col0 + 16nb + c]). */
+__device__ __forceinline__ void layer1(const uint16_t* __restrict__ w1, const float* step, const float xin[SRP_NN_IN],
+                                       int g, int c, f32x4 acc[4][4], __bf16* xt = nullptr, size_t ld = 0)
 {
 #pragma unroll
 	for (int mb = 0; mb < 4; mb++)
 #pragma unroll
 		for (int nb = 0; nb < 4; nb++)
 			acc[mb][nb] = zero4();
+#pragma unroll 1
 	for (int d = 0; d < SRP_NN_IN; d++)
 	{
-		float sd[4], cd[4];
+		/* xin[d] by selects (a dynamic index would put xin in memory) */
+		float own = xin[0];
+#pragma unroll
+		for (int t = 1; t < SRP_NN_IN; t++)
+			own = (d == t) ? xin[t] : own;
+		const float sdim = step[d];
+		float ud[4], sd[4], cd[4];
 #pragma unroll
 		for (int nb = 0; nb < 4; nb++)
-			sincosRev(SRP_NN_FREQ_STEP * x[nb][d], sd[nb], cd[nb]);
+		{
+			ud[nb] = sdim * __shfl(own, 16 * nb + c, 64);
+			sincosRev(ud[nb], sd[nb], cd[nb]);
+		}
 #pragma unroll
 		for (int half = 0; half < 2; half++)
 		{
@@ -164,7 +178,7 @@ __device__ __forceinline__ void layer1(const uint16_t* __restrict__ w1, const fl
 #pragma unroll
 			for (int nb = 0; nb < 4; nb++)
 			{
-				b[nb] = encode8(x[nb][d], j0, sd[nb], cd[nb]);
+				b[nb] = encode8(ud[nb], j0, sd[nb], cd[nb]);
 				if (xt)
 #pragma unroll
 					for (int q = 0; q < 8; q++)
@@ -219,16 +233,6 @@ __device__ __forceinline__ void layerOut(const uint16_t* __restrict__ w3f, const
 	}
 }
 
-/* Gather the features of the four fragments whose columns this lane feeds. */
-__device__ __forceinline__ void gatherFeatures(const float xin[SRP_NN_IN], int c, float x[4][8])
-{
-#pragma unroll
-	for (int nb = 0; nb < 4; nb++)
-#pragma unroll
-		for (int d = 0; d < SRP_NN_IN; d++)
-			x[nb][d] = __shfl(xin[d], 16 * nb + c, 64);
-}
-
 /* Channel ch of this lane's own fragment from the output tile. */
 __device__ __forceinline__ float ownOutput(const f32x4 y[4], int ch, int lane)
 {
@@ -250,10 +254,8 @@ __device__ inline void inferWave(const SrpNnDevice* __restrict__ net, const floa
 {
 	const int lane = (int) (threadIdx.x & 63u);
 	const int g = lane >> 4, c = lane & 15;
-	float x[4][8];
-	gatherFeatures(xin, c, x);
 	f32x4 acc[4][4];
-	layer1(net->w1, x, g, c, acc);
+	layer1(net->w1, net->step, xin, g, c, acc);
 	bf16x8 h[2][4];
 	hiddenOperands(acc, net->b1, g, h);
 	layerHidden(net->w2f, h, g, c, acc);
@@ -285,12 +287,13 @@ __device__ inline void inferLane(const SrpNnDevice* __restrict__ net, const floa
 		h1[m] = 0.0f;
 	for (int d = 0; d < SRP_NN_IN; d++)
 	{
+		const float u = net->step[d] * xin[d];
 		float sd, cd;
-		sincosRev(SRP_NN_FREQ_STEP * xin[d], sd, cd);
+		sincosRev(u, sd, cd);
 		for (int j0 = 0; j0 < SRP_NN_FREQ; j0 += 4)
 		{
 			float e[8];
-			encode4(xin[d], j0, sd, cd, e);
+			encode4(u, j0, sd, cd, e);
 			const int k0 = 64 * d + 2 * j0;
 #pragma unroll
 			for (int q = 0; q < 8; q++)

@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 }
 
 template <class FS, bool LATE, bool WORDS>
-__global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
+__global__ __launch_bounds__(64, 2) void srp_raster_batched(SrpRasterLaunch L)
 {
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
 	if (tile >= L.nTiles)
@@ -1023,7 +1023,7 @@ __global__ __launch_bounds__(64) void srp_raster_batched(SrpRasterLaunch L)
  * XCDs round-robin), and a wave that drains its own range moves on to the
  * next; otherwise one queue serves all. */
 template <class FS, bool LATE, bool WORDS>
-__global__ __launch_bounds__(64) void srp_raster_persistent(SrpRasterLaunch L)
+__global__ __launch_bounds__(64, 2) void srp_raster_persistent(SrpRasterLaunch L)
 {
 	const uint32_t nq = L.xcdQueues ? 8u : 1u;
 	const uint32_t q0 = blockIdx.x % nq;

@@ -4,7 +4,9 @@
  * Network: 8 surface features -> Fourier encoding (32 frequencies per
  * feature, sine and cosine: 512 values) -> 64 -> ReLU -> 64 -> ReLU -> 3.
  * Feature d at frequency j (0..31) contributes, at encoding index
- * 64*d + 2*j + s, sin (s = 0) or cos (s = 1) of 2*pi*SRP_NN_FREQ_STEP*(j+1)*x_d.
+ * 64*d + 2*j + s, sin (s = 0) or cos (s = 1) of 2*pi*(j+1)*u_d, where
+ * u_d = step_d * x_d and step_d (revolutions per unit of feature d) is a
+ * property of the network, SRP_NN_FREQ_STEP unless set otherwise.
  *
  * Parameters (fp32 master copy, "canonical" layout, row = output):
  *   W1[64][512] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]
@@ -52,4 +54,5 @@ typedef struct SrpNnDevice
 	const float* b1;
 	const float* b2;
 	const float* b3;
+	float step[SRP_NN_IN]; /* frequency step of each feature's encoding */
 } SrpNnDevice;

@@ -69,6 +69,12 @@ void srpNeuralNetGradients(SRPNeuralNet* net, const SRPNeuralDataset* ds, size_t
 void srpNeuralNetInfer(const SRPNeuralNet* net, const SRPNeuralDataset* ds, size_t count, float* rgb, int lanes);
 /** Device address of the network's SrpNnDevice view, for a uniform. */
 const void* srpNeuralNetDevice(const SRPNeuralNet* net);
+/** Frequency step of each of the SRP_NN_IN features' encodings (feature d
+ *  at frequency j enters as sin/cos of 2*pi*(j+1)*step[d]*x_d); every step
+ *  in (0, 1024), else an error message and no change.  A new network has
+ *  SRP_NN_FREQ_STEP for every feature.  Parameters are kept. */
+void srpNeuralNetSetEncoding(SRPNeuralNet* net, const float* step);
+void srpNeuralNetGetEncoding(const SRPNeuralNet* net, float* step);
 
 /* -------------------------------------------------------------- evaluation */
 

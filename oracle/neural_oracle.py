@@ -52,22 +52,24 @@ def pack(W1, b1, W2, b2, W3, b3):
     return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b3])
 
 
-def encode(F):
-    """[N, 8] features -> [N, 512] encoding (float64)."""
+def encode(F, steps=None):
+    """[N, 8] features -> [N, 512] encoding (float64); steps = frequency
+    step per feature (default FREQ_STEP for all)."""
     F = np.asarray(F, dtype=np.float64)
+    st = np.full(N_IN, FREQ_STEP) if steps is None else np.asarray(steps, dtype=np.float64)
     j = np.arange(N_FREQ, dtype=np.float64) + 1.0
-    a = 2.0 * np.pi * FREQ_STEP * F[:, :, None] * j[None, None, :]
+    a = 2.0 * np.pi * (st[None, :] * F)[:, :, None] * j[None, None, :]
     out = np.empty((F.shape[0], N_IN, N_FREQ, 2))
     out[..., 0] = np.sin(a)
     out[..., 1] = np.cos(a)
     return out.reshape(F.shape[0], N_ENC)
 
 
-def forward(p, F, bf=True):
+def forward(p, F, bf=True, steps=None):
     """Returns (y, cache) with y [N, 3]."""
     W1, b1, W2, b2, W3, b3 = unpack(p)
     r = bf16 if bf else (lambda v: np.asarray(v, dtype=np.float64))
-    X = r(encode(F))
+    X = r(encode(F, steps))
     z1 = X @ r(W1).T + b1
     h1 = r(np.maximum(z1, 0.0))
     z2 = h1 @ r(W2).T + b2
@@ -76,11 +78,11 @@ def forward(p, F, bf=True):
     return y, (X, z1, h1, z2, h2)
 
 
-def loss_and_grad(p, F, T, bf=True):
+def loss_and_grad(p, F, T, bf=True, steps=None):
     """Mean squared error over N*3 values and its gradient (canonical layout)."""
     W1, b1, W2, b2, W3, b3 = unpack(p)
     r = bf16 if bf else (lambda v: np.asarray(v, dtype=np.float64))
-    y, (X, z1, h1, z2, h2) = forward(p, F, bf)
+    y, (X, z1, h1, z2, h2) = forward(p, F, bf, steps)
     n = y.shape[0]
     e = y - np.asarray(T, dtype=np.float64)
     loss = float(np.mean(e * e))

@@ -362,10 +362,14 @@ __device__ bool setupTriangle(const SrpDrawParams* P, Vtx v0, Vtx v1, Vtx v2, ui
  * most P->maskBox (<= SRP_SMALL_BBOX) pixels on each side is scanned here exactly as the
  * reference scans it (raster/triangle.c:78-110: rows from the box origin,
  * dldy per row, dldx per pixel), giving one 64-bit coverage mask per 8x8
- * tile; tiles it does not cover get no entry.  Larger triangles are binned
- * to every tile of their box and their coverage is found by the rasteriser.
- * Calls emit(tileKey, mask, known) in row-major tile order. */
-template <class F>
+ * tile; tiles it does not cover get no entry (an empty entry would cost the
+ * rasteriser a window slot, and sliver fans such as a sphere's poles pile
+ * hundreds of them onto a few tiles).  Larger triangles are binned to every
+ * tile of their box and their coverage is found by the rasteriser.  Calls
+ * emit(tileKey, mask, known) in row-major tile order.  The counting pass
+ * (EMIT false) needs only which tiles are covered: once every tile of a
+ * band has a pixel, the band's remaining rows are stepped but not scanned. */
+template <bool EMIT, class F>
 __device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const SrpTri& T, uint32_t tileBase, F&& emit)
 {
 	if (T.x1 <= T.x0 || T.y1 <= T.y0)
@@ -382,14 +386,18 @@ __device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const Sr
 		 * reference's row-major scan of the whole box. */
 		float row[3] = {T.l0[0], T.l0[1], T.l0[2]};
 		int y = T.y0;
+		const uint32_t allCols = (1u << (tx1 - tx0 + 1u)) - 1u;
 		for (uint32_t ty = ty0; ty <= ty1; ty++)
 		{
 			uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
 			const int yEnd = ((int) (ty + 1u) << 3) < (int) T.y1 ? (int) (ty + 1u) << 3 : (int) T.y1;
 			for (; y < yEnd; y++)
 			{
+				const uint32_t found = (m0 != 0ull ? 1u : 0u) | (m1 != 0ull ? 2u : 0u) | (m2 != 0ull ? 4u : 0u) |
+				                       (m3 != 0ull ? 8u : 0u);
+				const int xEnd = (!EMIT && found == allCols) ? (int) T.x0 : (int) T.x1;
 				float l[3] = {row[0], row[1], row[2]};
-				for (int x = T.x0; x < (int) T.x1; x++)
+				for (int x = T.x0; x < xEnd; x++)
 				{
 					bool inside = true;
 					for (int e = 0; e < 3; e++)
@@ -600,7 +608,7 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const AsmDraw& W, cons
 				else
 					atomicOr(A.error, 2u);
 				uint32_t o = itemBase + nItems;
-				np = binTriangle(P, T, W.tileBase, [&](uint32_t key, uint64_t m, bool known) {
+				np = binTriangle<true>(P, T, W.tileBase, [&](uint32_t key, uint64_t m, bool known) {
 					if (o < A.itemCap)
 					{
 						A.pairKeys[o] = key;
@@ -614,17 +622,18 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const AsmDraw& W, cons
 					atomicOr(A.error, 2u);
 			}
 			else
-				np = binTriangle(P, T, W.tileBase, [](uint32_t, uint64_t, bool) {});
+				np = binTriangle<false>(P, T, W.tileBase, [](uint32_t, uint64_t, bool) {});
 			nRec++;
 			nItems += np;
 		}
 		else if (mode == SRP_POLYGON_MODE_LINE)
 		{
-			const Vtx* tv[3] = {&a, &b, &c};
-			for (int j = 0; j < 3; j++)
-			{
+			/* edges a-b, b-c, c-a; called by name, not through an array of
+			 * vertex pointers, which would put the vertices of every
+			 * triangle of the launch in scratch memory */
+			auto edge = [&](const Vtx& p, const Vtx& q) {
 				SrpLine L;
-				setupLine(P, *tv[j], *tv[(j + 1) % 3], recBase + nRec - idBase, L);
+				setupLine(P, p, q, recBase + nRec - idBase, L);
 				const uint32_t nf = lineFragCount(L);
 				if (EMIT)
 				{
@@ -633,19 +642,23 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const AsmDraw& W, cons
 				}
 				nRec++;
 				nItems += nf;
-			}
+			};
+			edge(a, b);
+			edge(b, c);
+			edge(c, a);
 		}
 		else
 		{
-			const Vtx* tv[3] = {&a, &b, &c};
-			for (int j = 0; j < 3; j++)
-			{
+			auto corner = [&](const Vtx& p) {
 				SrpPoint pt;
-				makePoint(P, *tv[j], recBase + nRec - idBase, pt);
+				makePoint(P, p, recBase + nRec - idBase, pt);
 				if (EMIT) A.points[recBase + nRec] = pt;
 				nItems += pointFrags<EMIT>(A, P, pt, recBase + nRec, itemBase + nItems);
 				nRec++;
-			}
+			};
+			corner(a);
+			corner(b);
+			corner(c);
 		}
 	}
 }
@@ -653,7 +666,7 @@ __device__ __forceinline__ void emitFan(const AsmArgs& A, const AsmDraw& W, cons
 /* A triangle that needs clipping: Sutherland-Hodgman over the pool, then
  * the varyings of the generated vertices that survive. */
 template <bool EMIT>
-__device__ __noinline__ void assembleClipped(const AsmArgs& A, const AsmDraw& W, const uint32_t* slot, uint32_t genBase,
+__device__ __forceinline__ void assembleClipped(const AsmArgs& A, const AsmDraw& W, const uint32_t* slot, uint32_t genBase,
                                              uint32_t recBase, uint32_t idBase, uint32_t itemBase,
                                              uint32_t& nRec, uint32_t& nGen, uint32_t& nItems)
 {
@@ -724,27 +737,96 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 		codeAnd &= c;
 		v[i].vref = slot[i];
 	}
-	uint32_t recBase = 0, genBase = 0, itemBase = 0, idBase = 0;
+	/* Triangles that need clipping are listed here and assembled by
+	 * k_assemble_clipped: the clipper's registers and stack would otherwise
+	 * be reserved by every wave of this kernel (148 against 59 VGPRs). */
+	const bool clip = codeOr != 0 && codeAnd == 0;
+	if (!EMIT)
+	{
+		const uint64_t want = __ballot(clip);
+		if (want)
+		{
+			const uint32_t lane = __lane_id();
+			const uint32_t leader = (uint32_t) __builtin_ctzll(want);
+			uint32_t at = 0;
+			if (lane == leader)
+				at = atomicAdd(A.clipCount, (uint32_t) __builtin_popcountll(want));
+			at = (uint32_t) __shfl((int) at, (int) leader);
+			if (clip)
+				A.clipList[at + (uint32_t) __builtin_popcountll(want & ((1ull << lane) - 1ull))] = g;
+		}
+	}
+	if (clip)
+		return;
+	uint32_t recBase = 0, itemBase = 0, idBase = 0;
 	if (EMIT)
 	{
 		recBase = A.recOff[g];
-		genBase = A.genOff[g];
 		itemBase = A.itemOff[g];
 		idBase = A.recOff[W.primBase];
 	}
-	uint32_t nRec = 0, nGen = 0, nItems = 0;
+	uint32_t nRec = 0, nItems = 0;
 	if (codeOr == 0)
 	{
 		for (int i = 0; i < 3; i++) v[i].invW = divide(v[i].c);
 		emitFan<EMIT, 3>(A, W, v, 3, recBase, idBase, itemBase, nRec, nItems);
 	}
-	else if (codeAnd == 0)
-		assembleClipped<EMIT>(A, W, slot, genBase, recBase, idBase, itemBase, nRec, nGen, nItems);
 	if (!EMIT)
 	{
 		A.recOff[g] = nRec;
-		A.genOff[g] = nGen;
+		A.genOff[g] = 0;
 		A.itemOff[g] = nItems;
+	}
+}
+
+/* The draw holding input g of a (possibly multi-draw) launch: the last
+ * draw whose first input is at or before it. */
+__device__ __forceinline__ uint32_t drawOfInput(const AsmDraw* D, uint32_t n, uint32_t g)
+{
+	uint32_t lo = 0, hi = n;
+	while (hi - lo > 1)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (D[mid].primBase <= g) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
+
+/* The triangles k_assemble_tris listed for clipping, in either pass (the
+ * list stays valid from the counting pass to the emitting one).  Each
+ * writes only at its own scanned offsets, so list order does not matter. */
+template <bool EMIT>
+__global__ __launch_bounds__(128) void k_assemble_clipped(AsmArgs A)
+{
+	const uint32_t n = *A.clipCount;
+	for (uint32_t i = blockIdx.x * 128u + threadIdx.x; i < n; i += gridDim.x * 128u)
+	{
+		const uint32_t g = A.clipList[i];
+		const uint32_t d = A.nDraws > 1 ? drawOfInput(A.draws, A.nDraws, g) : 0u;
+		const AsmDraw& W = A.draws[d];
+		const uint32_t k = g - W.primBase;
+		uint64_t s[3];
+		triStream(W.prim, W.start, k, s);
+		uint32_t slot[3];
+		for (int j = 0; j < 3; j++)
+			slot[j] = slotOf(W, s[j]);
+		uint32_t recBase = 0, genBase = 0, itemBase = 0, idBase = 0;
+		if (EMIT)
+		{
+			recBase = A.recOff[g];
+			genBase = A.genOff[g];
+			itemBase = A.itemOff[g];
+			idBase = A.recOff[W.primBase];
+		}
+		uint32_t nRec = 0, nGen = 0, nItems = 0;
+		assembleClipped<EMIT>(A, W, slot, genBase, recBase, idBase, itemBase, nRec, nGen, nItems);
+		if (!EMIT)
+		{
+			A.recOff[g] = nRec;
+			A.genOff[g] = nGen;
+			A.itemOff[g] = nItems;
+		}
 	}
 }
 
@@ -893,16 +975,31 @@ __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 	}
 }
 
-void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s)
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, bool clipPass)
 {
 	const uint32_t blocks = A.nBlocks;
 	if (!blocks) return;
 	switch (kind)
 	{
 		case 0:
-			if (emit) hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);
-			else hipLaunchKernelGGL(k_assemble_tris<false>, dim3(blocks), dim3(128), 0, s, A);
+		{
+			/* the clip list's length is known on the device only: a fixed
+			 * grid strides over it */
+			const uint32_t clipBlocks = blocks < 128u ? blocks : 128u;
+			if (emit)
+			{
+				hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);
+				if (clipPass)
+					hipLaunchKernelGGL(k_assemble_clipped<true>, dim3(clipBlocks), dim3(128), 0, s, A);
+			}
+			else
+			{
+				hipLaunchKernelGGL(k_assemble_tris<false>, dim3(blocks), dim3(128), 0, s, A);
+				if (clipPass)
+					hipLaunchKernelGGL(k_assemble_clipped<false>, dim3(clipBlocks), dim3(128), 0, s, A);
+			}
 			break;
+		}
 		case 1:
 			if (emit) hipLaunchKernelGGL(k_assemble_lines<true>, dim3(blocks), dim3(128), 0, s, A);
 			else hipLaunchKernelGGL(k_assemble_lines<false>, dim3(blocks), dim3(128), 0, s, A);

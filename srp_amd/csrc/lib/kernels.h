@@ -57,13 +57,18 @@ struct AsmArgs
 	uint32_t* fragKeys;
 	uint8_t* genVaryings;
 	uint32_t* error;      /* bit 0: clip overflow; bit 1: a capacity below was exceeded */
+	/* triangles (input slots) that need clipping, appended by the counting
+	 * pass; the counter starts at zero and has room for every input */
+	uint32_t* clipList;
+	uint32_t* clipCount;
 	/* capacities of the triangle outputs (records, tile pairs, generated
 	 * vertices); what lies beyond them is dropped and flagged */
 	uint32_t recCap, itemCap, genCap;
 };
 
-/* kind: 0 triangles, 1 lines, 2 points */
-void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s);
+/* kind: 0 triangles, 1 lines, 2 points.  clipPass = false skips the
+ * clipped-triangle kernel, for a caller that knows the clip list is empty. */
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, bool clipPass = true);
 
 /* util.hip */
 struct ScanScratch

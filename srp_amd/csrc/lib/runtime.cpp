@@ -1156,7 +1156,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	vse->launch(&vl, (void*) R.stream);
 
 	/* Assembly COUNT pass + scans. */
-	uint32_t* counts = R.counts.get<uint32_t>((nIn + 1) * 3 * sizeof(uint32_t));
+	uint32_t* counts = R.counts.get<uint32_t>((nIn + 1) * 4 * sizeof(uint32_t));
 	uint32_t* totals = R.totals.get<uint32_t>(8 * sizeof(uint32_t));
 	fillU32(totals, 0u, 8, R.stream);
 	AsmArgs A;
@@ -1167,14 +1167,17 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	A.recOff = counts;
 	A.genOff = counts + (nIn + 1);
 	A.itemOff = counts + 2 * (nIn + 1);
+	A.clipList = counts + 3 * (nIn + 1);
 	A.error = totals + 3;
+	A.clipCount = totals + 4;
 	A.recCap = A.itemCap = A.genCap = 0xFFFFFFFFu;
 	launchAssemble(A, kind, false, R.stream);
 	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream);
 	if (R.profile) R.span(3, evDraw, R.mark());
-	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 	const uint32_t nRec = R.hTotals[0], nGen = R.hTotals[1], nItems = R.hTotals[2];
+	const bool anyClipped = R.hTotals[4] != 0;
 	if (R.hTotals[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
 		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
@@ -1210,7 +1213,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		A.frags = R.frags.get<SrpFrag>((size_t) nItems * sizeof(SrpFrag) + 16);
 		A.fragKeys = R.fragKeys.get<uint32_t>((size_t) nItems * 4 + 16);
 	}
-	launchAssemble(A, kind, true, R.stream);
+	launchAssemble(A, kind, true, R.stream, anyClipped);
 
 	if (backend == 0)
 	{
@@ -1384,6 +1387,8 @@ AsmArgs batchAsmArgs(SRPBatch* B)
 	A.recOff = B->counts;
 	A.genOff = B->counts + (B->nIn + 1);
 	A.itemOff = B->counts + 2 * ((size_t) B->nIn + 1);
+	A.clipList = B->counts + 3 * ((size_t) B->nIn + 1);
+	A.clipCount = B->totals + 4;
 	A.tris = B->tris;
 	A.pairKeys = B->pairKeys;
 	A.pairVals = B->pairVals;
@@ -1603,7 +1608,7 @@ SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
 	}
 	HIP_CHECK(hipMalloc((void**) &B->clip, clipSlots * 16 + 16));
 	HIP_CHECK(hipMalloc((void**) &B->varyings, varBytes + 512));
-	HIP_CHECK(hipMalloc((void**) &B->counts, ((size_t) B->nIn + 1) * 3 * sizeof(uint32_t)));
+	HIP_CHECK(hipMalloc((void**) &B->counts, ((size_t) B->nIn + 1) * 4 * sizeof(uint32_t)));
 	B->hDraws.resize(nd);
 	B->hVsSeg.resize(nd);
 	B->hRasterSeg.resize(nd);

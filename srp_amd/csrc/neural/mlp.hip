@@ -113,10 +113,8 @@ __global__ __launch_bounds__(64) void k_train_pass(TrainArgs A)
 	const size_t ld = A.batch;
 
 	/* forward */
-	float x[4][8];
-	gatherFeatures(xin, c, x);
 	f32x4 z1[4][4];
-	layer1(net->w1, x, g, c, z1, A.xt + s0, ld);
+	layer1(net->w1, net->step, xin, g, c, z1, A.xt + s0, ld);
 	biasInPlace(z1, net->b1, g);
 	storeTile(A.h1t + s0, ld, z1, g, c, true);
 	bf16x8 h[2][4];
@@ -480,6 +478,7 @@ struct SRPNeuralNet
 	float* m = nullptr;
 	float* v = nullptr;
 	SrpNnDevice* view = nullptr;   /* device */
+	SrpNnDevice host;              /* host copy of *view */
 	int adamSteps = 0;
 	/* training buffers, sized for kMaxBatch */
 	__bf16 *xt = nullptr, *h1t = nullptr, *h2t = nullptr, *dz1t = nullptr, *dz2t = nullptr, *dyt = nullptr,
@@ -732,6 +731,9 @@ SRPNeuralNet* srpNewNeuralNet(uint64_t seed)
 	view.b1 = N->M.w + SRP_NN_OFF_B1;
 	view.b2 = N->M.w + SRP_NN_OFF_B2;
 	view.b3 = N->M.w + SRP_NN_OFF_B3;
+	for (int d = 0; d < SRP_NN_IN; d++)
+		view.step[d] = SRP_NN_FREQ_STEP;
+	N->host = view;
 	N->view = devAlloc<SrpNnDevice>(1);
 	NN_CHECK(hipMemcpy(N->view, &view, sizeof(view), hipMemcpyHostToDevice));
 
@@ -773,6 +775,28 @@ void srpNeuralNetGetParams(const SRPNeuralNet* N, float* params)
 void srpNeuralNetSetParams(SRPNeuralNet* N, const float* params) { uploadParams(N, params); }
 
 const void* srpNeuralNetDevice(const SRPNeuralNet* N) { return N ? (const void*) N->view : NULL; }
+
+void srpNeuralNetSetEncoding(SRPNeuralNet* N, const float* step)
+{
+	for (int d = 0; d < SRP_NN_IN; d++)
+		if (!(step[d] > 0.0f && step[d] < 1024.0f))
+		{
+			libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__,
+			           "frequency step %g of feature %d is not in (0, 1024)", (double) step[d], d);
+			return;
+		}
+	hipStream_t s = libStream();
+	NN_CHECK(hipStreamSynchronize(s));
+	for (int d = 0; d < SRP_NN_IN; d++)
+		N->host.step[d] = step[d];
+	NN_CHECK(hipMemcpy(N->view, &N->host, sizeof(SrpNnDevice), hipMemcpyHostToDevice));
+}
+
+void srpNeuralNetGetEncoding(const SRPNeuralNet* N, float* step)
+{
+	for (int d = 0; d < SRP_NN_IN; d++)
+		step[d] = N->host.step[d];
+}
 
 void srpNeuralNetTrain(SRPNeuralNet* N, const SRPNeuralDataset* D, int steps, int batch, float lr0, float lr1,
                        uint64_t seed, float* lossOut)

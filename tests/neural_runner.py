@@ -44,6 +44,8 @@ class Neural:
         self._grad = _sig(L, "srpNeuralNetGradients", None, [_vp, _vp, _c.c_size_t, _f32p, _vp])
         self._infer = _sig(L, "srpNeuralNetInfer", None, [_vp, _vp, _c.c_size_t, _f32p, _c.c_int])
         self.net_device = _sig(L, "srpNeuralNetDevice", _vp, [_vp])
+        self._set_enc = _sig(L, "srpNeuralNetSetEncoding", None, [_vp, _f32p])
+        self._get_enc = _sig(L, "srpNeuralNetGetEncoding", None, [_vp, _f32p])
         self._compare = _sig(L, "srpNeuralCompare", None, [_vp, _vp, _vp, _vp, _vp, _vp])
         self.sync = _sig(L, "srpHipSynchronize", None, [])
         self.profile_enable = _sig(L, "srpHipProfileEnable", None, [_c.c_bool])
@@ -82,6 +84,14 @@ class Neural:
 
     def set_params(self, net, p):
         self._set(net, np.ascontiguousarray(p, np.float32))
+
+    def set_encoding(self, net, steps):
+        self._set_enc(net, np.ascontiguousarray(steps, np.float32))
+
+    def get_encoding(self, net):
+        st = np.empty(8, np.float32)
+        self._get_enc(net, st)
+        return st
 
     def train(self, net, ds, steps, batch, lr0, lr1, seed):
         loss = np.empty(steps, np.float32)

@@ -79,6 +79,39 @@ def test_infer_matches_oracle(nn, scale):
         nn.free_ds(ds)
 
 
+def test_encoding_steps_match_oracle(nn):
+    """Per-feature frequency steps set on the network reach the inference
+    paths and the gradient exactly as the oracle's encoding applies them."""
+    n = 2048
+    F, T = synth(n, 6)
+    st = np.array([1.5, 0.5, 0.75, 0.5, 0.5, 0.5, 3.0, 1.25], np.float32)
+    ds = nn.new_ds(n)
+    net = nn.new_net(8)
+    try:
+        nn.ds_write(ds, F, T)
+        nn.set_encoding(net, st)
+        assert np.array_equal(nn.get_encoding(net), st)
+        p = (no.init_params(8) * 2.0).astype(np.float32)
+        nn.set_params(net, p)
+        yo, _ = no.forward(p.astype(np.float64), F, bf=True, steps=st)
+        tol = 1e-2 * max(float(np.abs(yo).mean()), 0.2)
+        for lanes in (False, True):
+            e = np.abs(nn.infer(net, ds, n, lanes=lanes) - yo)
+            print(f"[neural] steps, lanes={lanes}: max {e.max():.2e} mean {e.mean():.2e}", flush=True)
+            assert e.max() < tol and e.mean() < tol / 20
+        g, loss = nn.gradients(net, ds, n)
+        lo, go = no.loss_and_grad(p.astype(np.float64), F, T, bf=True, steps=st)
+        assert abs(loss - lo) <= 1e-4 * lo
+        rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+        assert rel < 1e-2
+        # a step outside (0, 1024) is refused and changes nothing
+        nn.set_encoding(net, np.full(8, -1.0, np.float32))
+        assert np.array_equal(nn.get_encoding(net), st)
+    finally:
+        nn.free_net(net)
+        nn.free_ds(ds)
+
+
 def test_gradients_match_oracle(nn):
     n = 2048
     F, T = synth(n, 2)

@@ -110,6 +110,26 @@ def test_encoding_closed_forms():
     assert np.allclose(Ep[..., 0], -Em[..., 0]) and np.allclose(Ep[..., 1], Em[..., 1])
 
 
+def test_encoding_per_feature_steps():
+    """A frequency step per feature scales that feature alone: encoding with
+    steps s equals the default encoding of the features times s/FREQ_STEP,
+    and a quarter-revolution input gives the closed form per feature."""
+    rng = np.random.default_rng(11)
+    F = rng.uniform(-1, 1, (6, 8))
+    st = np.array([0.5, 1.5, 0.75, 0.25, 2.0, 1.0, 3.0, 0.5])
+    assert np.allclose(no.encode(F, st), no.encode(F * st / no.FREQ_STEP), atol=1e-12)
+    E = no.encode(np.tile(0.25 / st, (1, 1)), st).reshape(1, 8, 32, 2)
+    j = np.arange(32) + 1
+    for d in range(8):
+        assert np.allclose(E[0, d, :, 0], np.sin(np.pi * j / 2), atol=1e-12)
+        assert np.allclose(E[0, d, :, 1], np.cos(np.pi * j / 2), atol=1e-12)
+    # a different step on one feature changes only that feature's block
+    st2 = st.copy()
+    st2[3] = 0.6
+    d = np.abs(no.encode(F, st) - no.encode(F, st2)).reshape(6, 8, 64).max(axis=(0, 2))
+    assert d[3] > 1e-3 and np.all(np.delete(d, 3) == 0)
+
+
 def test_sampler_is_deterministic_and_in_range():
     a = no.batch_rows(9, 3, 256, 1000)
     b = no.batch_rows(9, 3, 256, 1000)
