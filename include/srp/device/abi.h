@@ -19,6 +19,11 @@
  * SRP_STAT_STRIDE u64 (one 128-byte line each), slot s holding covered at
  * [s * STRIDE] and shaded at [s * STRIDE + 1]; waves add to slot
  * (block % SLOTS) so that their atomics do not all meet on one address. */
+/* The largest float not above the reference's EPSILON (1e-9, as a double;
+ * math/utils.h:45 tests fabs(x) <= EPSILON on a float promoted to double).
+ * For every float x, |x| <= 1e-9 holds exactly when |x| <= this, so the
+ * test needs no double arithmetic. */
+#define SRP_EPSILON_F 0x1.12e0bep-30f
 #define SRP_STAT_SLOTS 64
 #define SRP_STAT_STRIDE 16
 

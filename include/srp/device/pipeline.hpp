@@ -27,7 +27,7 @@ namespace srp {
 namespace dev {
 
 /* reference math/utils.h:41-45 (comparison in double precision) */
-__device__ __forceinline__ bool roughlyZero(float x) { return fabs((double) x) <= 1e-9; }
+__device__ __forceinline__ bool roughlyZero(float x) { return fabsf(x) <= SRP_EPSILON_F; }
 
 /* reference raster/fragment.c:156-178, :225-245 */
 template <typename T>

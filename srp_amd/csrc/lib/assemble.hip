@@ -975,7 +975,7 @@ __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 	}
 }
 
-void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, bool clipPass)
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped)
 {
 	const uint32_t blocks = A.nBlocks;
 	if (!blocks) return;
@@ -983,19 +983,26 @@ void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, bool c
 	{
 		case 0:
 		{
-			/* the clip list's length is known on the device only: a fixed
-			 * grid strides over it */
-			const uint32_t clipBlocks = blocks < 128u ? blocks : 128u;
+			/* The clip list's length is on the device; a grid of up to 1024
+			 * blocks strides over it (each clipped triangle is a long serial
+			 * job, so enough threads for one each matters more than the cost
+			 * of a grid that finds the list empty). */
+			uint32_t clipBlocks = blocks < 1024u ? blocks : 1024u;
+			if (clipped != kClipUnknown)
+			{
+				const uint32_t need = (uint32_t) (((uint64_t) clipped + 127u) / 128u);
+				clipBlocks = need < clipBlocks ? need : clipBlocks;
+			}
 			if (emit)
 			{
 				hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);
-				if (clipPass)
+				if (clipBlocks)
 					hipLaunchKernelGGL(k_assemble_clipped<true>, dim3(clipBlocks), dim3(128), 0, s, A);
 			}
 			else
 			{
 				hipLaunchKernelGGL(k_assemble_tris<false>, dim3(blocks), dim3(128), 0, s, A);
-				if (clipPass)
+				if (clipBlocks)
 					hipLaunchKernelGGL(k_assemble_clipped<false>, dim3(clipBlocks), dim3(128), 0, s, A);
 			}
 			break;

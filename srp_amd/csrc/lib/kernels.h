@@ -66,9 +66,11 @@ struct AsmArgs
 	uint32_t recCap, itemCap, genCap;
 };
 
-/* kind: 0 triangles, 1 lines, 2 points.  clipPass = false skips the
- * clipped-triangle kernel, for a caller that knows the clip list is empty. */
-void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, bool clipPass = true);
+/* kind: 0 triangles, 1 lines, 2 points.  clipped: how many triangles the
+ * counting pass listed for clipping, when the caller knows it (0 skips the
+ * clipped-triangle kernel); kClipUnknown sizes its grid for the worst case. */
+constexpr uint32_t kClipUnknown = 0xFFFFFFFFu;
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped = kClipUnknown);
 
 /* util.hip */
 struct ScanScratch

@@ -1121,7 +1121,9 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.cacheSize = cacheSize;
 	H.clip = R.clip.get<float>((size_t) cacheSize * 16);
 	H.varyings = R.varyings.get<uint8_t>((size_t) cacheSize * V->varyingsSize + 512);
-	H.genVaryings = H.varyings;
+	/* the buffer of clip-generated vertices as it stands; re-sent after the
+	 * counting pass only if that pass needs it larger */
+	H.genVaryings = R.genVaryings.get<uint8_t>(512);
 	H.tileShift = (R.tile == 16) ? 4u : 3u;
 	H.maskBox = R.maskBox;
 	H.tilesX = (H.width + R.tile - 1) / R.tile;
@@ -1177,7 +1179,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 	const uint32_t nRec = R.hTotals[0], nGen = R.hTotals[1], nItems = R.hTotals[2];
-	const bool anyClipped = R.hTotals[4] != 0;
+	const uint32_t nClipped = R.hTotals[4];
 	if (R.hTotals[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
 		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
@@ -1188,8 +1190,12 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 
 	/* EMIT pass. */
 	const int backend = (kind == 0) ? (int) S.polygonMode : (kind == 1 ? 1 : 2);  /* 0 tiles, 1 lines, 2 points */
-	H.genVaryings = R.genVaryings.get<uint8_t>((size_t) nGen * V->varyingsSize + 512);
-	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
+	uint8_t* genBuf = R.genVaryings.get<uint8_t>((size_t) nGen * V->varyingsSize + 512);
+	if (genBuf != H.genVaryings)
+	{
+		H.genVaryings = genBuf;
+		HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
+	}
 	HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
 	A.genVaryings = H.genVaryings;
 	const uint32_t nTiles = H.tilesX * H.tilesY;
@@ -1213,7 +1219,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		A.frags = R.frags.get<SrpFrag>((size_t) nItems * sizeof(SrpFrag) + 16);
 		A.fragKeys = R.fragKeys.get<uint32_t>((size_t) nItems * 4 + 16);
 	}
-	launchAssemble(A, kind, true, R.stream, anyClipped);
+	launchAssemble(A, kind, true, R.stream, nClipped);
 
 	if (backend == 0)
 	{

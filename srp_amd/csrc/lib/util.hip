@@ -211,8 +211,16 @@ void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, 
 constexpr int RS_THREADS = 256;
 constexpr int RS_CHUNK = 4096;   /* items per block */
 
+/* Up to this many blocks, each scatter block sums the per-block digit counts
+ * itself (reading all of them from L2) instead of a separate scan over them:
+ * two launches per pass fewer, at nBlocks KiB of L2 reads per block. */
+constexpr uint32_t RS_FUSED_MAX_BLOCKS = 256;
+
+/* Digit counts of one block of RS_CHUNK keys, written at
+ * hist[digit * digitStride + block * blockStride]. */
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys, size_t n, const uint32_t* nDev,
-                                                           uint32_t shift, uint32_t* hist, uint32_t nBlocks)
+                                                           uint32_t shift, uint32_t* hist, uint32_t digitStride,
+                                                           uint32_t blockStride)
 {
 	if (nDev && *nDev < n) n = *nDev;
 	__shared__ uint32_t h[256];
@@ -224,11 +232,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys,
 	for (size_t i = begin + t; i < end; i += RS_THREADS)
 		atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
 	__syncthreads();
-	hist[(size_t) t * nBlocks + blockIdx.x] = h[t];
+	hist[(size_t) t * digitStride + (size_t) blockIdx.x * blockStride] = h[t];
 }
 
 /* Stable scatter: items keep their input order within a digit.  Each tile
- * of 256 items ranks its items per digit with wave ballots, then by wave. */
+ * of 256 items ranks its items per digit with wave ballots, then by wave.
+ * FUSED: hist holds raw block-major counts (hist[block * 256 + digit]) and
+ * the block finds its digits' offsets itself; otherwise hist holds the
+ * exclusive scan of the digit-major counts. */
+template <bool FUSED>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
                                                               uint32_t* keysOut, uint32_t* valsOut, size_t n,
                                                               const uint32_t* nDev, uint32_t shift, const uint32_t* hist,
@@ -240,7 +252,54 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 	__shared__ uint32_t tileTotal[256];
 	const uint32_t t = threadIdx.x;
 	const uint32_t lane = t & 63u, wave = t >> 6;
-	running[t] = hist[(size_t) t * nBlocks + blockIdx.x];
+	if (FUSED)
+	{
+		/* Digit t: items of all blocks (total) and of the blocks before
+		 * this one (before).  Wave w reads every fourth block row, a lane
+		 * four digits at once; the four waves' sums meet in LDS. */
+		__shared__ uint32_t befCnt[4][256];
+		const uint32_t d4 = 4u * lane;
+		uint32_t tot[4] = {0u, 0u, 0u, 0u}, bef[4] = {0u, 0u, 0u, 0u};
+#pragma unroll 8
+		for (uint32_t b = wave; b < nBlocks; b += 4u)
+		{
+			const uint4 v = *reinterpret_cast<const uint4*>(hist + (size_t) b * 256u + d4);
+			const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+			const uint32_t take = (b < blockIdx.x) ? ~0u : 0u;
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+			{
+				tot[k] += vv[k];
+				bef[k] += vv[k] & take;
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+		{
+			waveCnt[wave][d4 + k] = tot[k];
+			befCnt[wave][d4 + k] = bef[k];
+		}
+		__syncthreads();
+		const uint32_t total = waveCnt[0][t] + waveCnt[1][t] + waveCnt[2][t] + waveCnt[3][t];
+		const uint32_t before = befCnt[0][t] + befCnt[1][t] + befCnt[2][t] + befCnt[3][t];
+		/* exclusive scan of the totals over the 256 digits */
+		uint32_t incl = total;
+		for (int off = 1; off < 64; off <<= 1)
+		{
+			const uint32_t v = __shfl_up(incl, off);
+			if ((int) lane >= off) incl += v;
+		}
+		if (lane == 63u)
+			tileTotal[wave] = incl;
+		__syncthreads();   /* also: every total read before waveCnt is reused below */
+		uint32_t waveBase = 0;
+		for (uint32_t w = 0; w < wave; w++)
+			waveBase += tileTotal[w];
+		running[t] = waveBase + incl - total + before;
+		__syncthreads();
+	}
+	else
+		running[t] = hist[(size_t) t * nBlocks + blockIdx.x];
 	const size_t begin = (size_t) blockIdx.x * RS_CHUNK;
 	const size_t end = (begin + RS_CHUNK < n) ? begin + RS_CHUNK : n;
 	const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
@@ -318,13 +377,24 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 	uint32_t* vIn = vals;
 	uint32_t* kOut = sc.keys;
 	uint32_t* vOut = sc.vals;
+	const bool fused = nBlocks <= RS_FUSED_MAX_BLOCKS;
 	for (int shift = 0; shift < bits; shift += 8)
 	{
-		hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift, sc.hist,
-		                   nBlocks);
-		scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
-		hipLaunchKernelGGL(k_radix_scatter, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n, nDev,
-		                   (uint32_t) shift, sc.hist, nBlocks);
+		if (fused)
+		{
+			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
+			                   sc.hist, 1u, 256u);
+			hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks);
+		}
+		else
+		{
+			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
+			                   sc.hist, nBlocks, 1u);
+			scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
+			hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks);
+		}
 		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
 		uint32_t* tv = vIn; vIn = vOut; vOut = tv;
 	}
