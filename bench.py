@@ -261,7 +261,7 @@ def batch_leg(args):
         tuned_ms, tuning = gb.autotune(3)
         out["autotune"] = {"s": round(time.perf_counter() - t0, 3), "best_ms": round(tuned_ms, 4),
                            "waves_per_cu": tuning[0], "mask_box": tuning[1], "xcd_queues": tuning[2],
-                           "pixel_runs": tuning[3], "candidates": 36}
+                           "pixel_runs": tuning[3], "candidates": 48}
         eager_ms, shaded = timed(lambda: gb.frame(True), 10)
         assert gb.capture() == 0
         graph_ms, g_shaded = timed(lambda: gb.frame(True), args.batch_frames)

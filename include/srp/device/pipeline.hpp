@@ -463,7 +463,18 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
  *   C  resolve, lane = pixel: stencil, depth and colour applied in
  *      primitive order exactly as fragment.c:63-125 does, counting a
  *      fragment as shaded only where the reference invokes the shader. */
-constexpr int RF = 256;
+constexpr int RF = 128;
+
+/* Waves per SIMD the batched rasteriser kernels are compiled for (register
+ * budget 512 / waves, so 72 VGPRs at 7).  Tiles are latency-bound and a
+ * 1080p frame has a few thousand busy tiles, so the more of them resident
+ * at once the better; 128-fragment batches keep LDS at 4.7 KB per wave.
+ * Measured on the 1080p mesh: 5 waves 71.9 us, 6 71.6, 7 62.8, 8 70.1 (the
+ * 64-register budget spills).  A wave-cooperative shader (a network on the
+ * matrix cores) needs more than 128 registers and gets 2. */
+#ifndef SRP_RASTER_WAVES
+#define SRP_RASTER_WAVES 7
+#endif
 
 /* Fragment shaders that evaluate a whole wave of fragments together (all 64
  * lanes call FS().wave(in, out, active) convergently) declare
@@ -494,7 +505,10 @@ struct VaryingBuf<false>
 	static constexpr uint32_t n = SRP_MAX_VARYING_BYTES / 4;
 };
 
-enum : uint8_t { FRAG_SCISSORED = 1, FRAG_CULLED = 2, FRAG_SHADED = 3 };
+/* Per-fragment state kept in LDS between shading and resolve; FRAG_FRONT
+ * is or'ed in for fragments of front-facing triangles, so the resolve
+ * reads no triangle record. */
+enum : uint8_t { FRAG_SCISSORED = 1, FRAG_CULLED = 2, FRAG_SHADED = 3, FRAG_FRONT = 0x80 };
 
 __device__ __forceinline__ bool monotoneOp(uint32_t op)
 {
@@ -865,6 +879,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			const uint32_t p = (f < nFrag) ? fragPix[f] : 0u;
 			const float pixDepth = __shfl(dep, (int) p);
 			uint8_t state = FRAG_CULLED;
+			uint8_t facing = 0;   /* FRAG_FRONT when the fragment's triangle is front facing */
 			float depth = 0.0f;
 			SRPFragmentShaderIn in;
 			SRPFragmentShaderOut out;
@@ -877,6 +892,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			{
 				const uint32_t b = fragTri[f];
 				const SrpTri& T = L.tris[recIdx[(head + b) & 63u]];
+				facing = ((T.flags >> 3) & 1u) ? FRAG_FRONT : 0;
 				const int fx = (int) (tx * 8 + (p & 7u)), fy = (int) (ty * 8 + (p >> 3));
 				float l[3];
 				if ((knownLanes >> b) & 1ull)
@@ -951,7 +967,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 					fragCol[f] = packColor(out.color);
 				}
 				fragDep[f] = depth;
-				fragState[f] = state;
+				fragState[f] = state | facing;
 			}
 		}
 		__syncthreads();
@@ -966,10 +982,10 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			pm &= pm - 1ull;
 			const uint32_t f = fragBase[b] + (uint32_t) __popcll(triMask[b] & ltMask);
 			covered++;
-			const uint8_t state = fragState[f];
+			const uint8_t state = fragState[f] & (uint8_t) ~FRAG_FRONT;
 			if (state == FRAG_SCISSORED)
 				continue;
-			const bool front = (L.tris[recIdx[(head + b) & 63u]].flags >> 3) & 1u;
+			const bool front = (fragState[f] & FRAG_FRONT) != 0;
 			const SRPStencilFaceState& sf = st.stencil[front ? 0 : 1];
 			if (st.stencilEnabled && !compare<uint32_t>(sf.func, (uint32_t) (sf.ref & sf.mask), sten & sf.mask))
 			{
@@ -1005,8 +1021,8 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	}
 }
 
-template <class FS, bool LATE, bool WORDS>
-__global__ __launch_bounds__(64, 2) void srp_raster_batched(SrpRasterLaunch L)
+template <class FS, bool LATE, bool WORDS, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void srp_raster_batched(SrpRasterLaunch L)
 {
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
 	if (tile >= L.nTiles)
@@ -1022,8 +1038,8 @@ __global__ __launch_bounds__(64, 2) void srp_raster_batched(SrpRasterLaunch L)
  * split into 8 contiguous ranges, one per XCD (blocks are dispatched to
  * XCDs round-robin), and a wave that drains its own range moves on to the
  * next; otherwise one queue serves all. */
-template <class FS, bool LATE, bool WORDS>
-__global__ __launch_bounds__(64, 2) void srp_raster_persistent(SrpRasterLaunch L)
+template <class FS, bool LATE, bool WORDS, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void srp_raster_persistent(SrpRasterLaunch L)
 {
 	const uint32_t nq = L.xcdQueues ? 8u : 1u;
 	const uint32_t q0 = blockIdx.x % nq;
@@ -1154,10 +1170,11 @@ void launchRasterT(const SrpRasterLaunch* L, hipStream_t s)
 template <class FS, bool LATE, bool WORDS>
 void launchBatchedT(const SrpRasterLaunch* L, hipStream_t s)
 {
+	constexpr int W = FsIsWave<FS>::value ? 2 : SRP_RASTER_WAVES;
 	if (L->nSeg)
-		hipLaunchKernelGGL((srp_raster_persistent<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+		hipLaunchKernelGGL((srp_raster_persistent<FS, LATE, WORDS, W>), dim3(L->nBlocks), dim3(64), 0, s, *L);
 	else
-		hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+		hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS, W>), dim3(L->nBlocks), dim3(64), 0, s, *L);
 }
 
 template <class FS>

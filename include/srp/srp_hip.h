@@ -129,8 +129,8 @@ int srpBatchCapture(SRPBatch* batch);
 /** Change the tuning (drops a captured graph) / read it. */
 void srpBatchSetTuning(SRPBatch* batch, const SRPBatchTuning* tuning);
 void srpBatchGetTuning(const SRPBatch* batch, SRPBatchTuning* tuning);
-/** Time `trials` frames of every candidate tuning (36: waves per CU 4, 8,
- *  16 x mask box 0, 16, 24 x both queue modes x both run modes) and keep
+/** Time `trials` frames of every candidate tuning (48: waves per CU 4, 8,
+ *  16, 28 x mask box 0, 16, 24 x both queue modes x both run modes) and keep
  *  the fastest;
  *  returns its mean frame time in ms and writes it to *best (if not NULL). */
 double srpBatchAutotune(SRPBatch* batch, int trials, SRPBatchTuning* best);

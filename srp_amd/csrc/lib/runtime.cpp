@@ -1794,7 +1794,7 @@ double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 		return 0.0;
 	if (trials < 1) trials = 1;
 	Runtime& R = rt();
-	const int waves[] = {4, 8, 16};
+	const int waves[] = {4, 8, 16, 28};
 	const int boxes[] = {0, 16, 24};
 	const int queues[] = {0, 1};
 	const int runs[] = {0, 1};
