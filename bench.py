@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--lines", type=int, default=100000)
     ap.add_argument("--points", type=int, default=100000)
     ap.add_argument("--tile", type=int, default=0, help="raster tile side (8 or 16); 0 = library default")
+    ap.add_argument("--mask-box", type=int, default=-1,
+                    help="largest triangle box side given exact binning masks (0..24); -1 = library default")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid-x", type=int, default=1000, help="terrain cells along x (neural leg)")
@@ -365,6 +367,9 @@ def main():
     gpu = load(GPU_SO)
     if args.tile:
         gpu.srpHipSetTileSize(args.tile)
+    if args.mask_box >= 0:
+        gpu.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
+        gpu.srpHipSetRasterOptions(-1, -1, args.mask_box)
     setup(gpu, args, tex)
     for _ in range(args.warmup):
         gpu.srpBenchFrame()
