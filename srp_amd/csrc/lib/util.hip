@@ -209,6 +209,7 @@ void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, 
 /* ------------------------------------------------------------ radix sort */
 
 constexpr int RS_THREADS = 256;
+constexpr int RS_SLICES = 2;     /* slices of RS_THREADS items ranked per round */
 constexpr int RS_CHUNK = 4096;   /* items per block */
 
 /* Up to this many blocks, each scatter block sums the per-block digit counts
@@ -248,8 +249,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 {
 	if (nDev && *nDev < n) n = *nDev;
 	__shared__ uint32_t running[256];
-	__shared__ uint32_t waveCnt[4][256];
+	__shared__ uint32_t sliceCnt[RS_SLICES * 4][256];
 	__shared__ uint32_t tileTotal[256];
+	uint32_t (*waveCnt)[256] = sliceCnt;   /* the offsets prologue's per-wave sums */
 	const uint32_t t = threadIdx.x;
 	const uint32_t lane = t & 63u, wave = t >> 6;
 	if (FUSED)
@@ -303,48 +305,63 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 	const size_t begin = (size_t) blockIdx.x * RS_CHUNK;
 	const size_t end = (begin + RS_CHUNK < n) ? begin + RS_CHUNK : n;
 	const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
-	for (size_t base = begin; base < end; base += RS_THREADS)
+	/* Rounds of RS_SLICES slices of RS_THREADS items; item order is slice,
+	 * then wave, then lane, and so is the order of the per-(slice, wave)
+	 * digit counts summed below. */
+	for (size_t base = begin; base < end; base += (size_t) RS_SLICES * RS_THREADS)
 	{
-		for (int w = 0; w < 4; w++) waveCnt[w][t] = 0;
+		for (int k = 0; k < RS_SLICES * 4; k++) sliceCnt[k][t] = 0;
 		__syncthreads();
-		const size_t i = base + t;
-		const bool valid = i < end;
-		uint32_t key = 0, val = 0, d = 0;
-		if (valid)
-		{
-			key = keysIn[i];
-			val = valsIn[i];
-			d = (key >> shift) & 255u;
-		}
-		uint64_t peers = __ballot(valid);
+		uint32_t key[RS_SLICES], val[RS_SLICES], d[RS_SLICES], rank[RS_SLICES];
+		bool valid[RS_SLICES];
 #pragma unroll
-		for (int b = 0; b < 8; b++)
+		for (int sl = 0; sl < RS_SLICES; sl++)
 		{
-			const bool bit = (d >> b) & 1u;
-			const uint64_t m = __ballot(bit);
-			peers &= bit ? m : ~m;
+			const size_t i = base + (size_t) sl * RS_THREADS + t;
+			valid[sl] = i < end;
+			key[sl] = 0; val[sl] = 0; d[sl] = 0;
+			if (valid[sl])
+			{
+				key[sl] = keysIn[i];
+				val[sl] = valsIn[i];
+				d[sl] = (key[sl] >> shift) & 255u;
+			}
 		}
-		const uint32_t rank = (uint32_t) __popcll(peers & ltMask);
-		if (valid && rank == 0)
-			waveCnt[wave][d] = (uint32_t) __popcll(peers);
+#pragma unroll
+		for (int sl = 0; sl < RS_SLICES; sl++)
+		{
+			uint64_t peers = __ballot(valid[sl]);
+#pragma unroll
+			for (int b = 0; b < 8; b++)
+			{
+				const bool bit = (d[sl] >> b) & 1u;
+				const uint64_t m = __ballot(bit);
+				peers &= bit ? m : ~m;
+			}
+			rank[sl] = (uint32_t) __popcll(peers & ltMask);
+			if (valid[sl] && rank[sl] == 0)
+				sliceCnt[sl * 4 + wave][d[sl]] = (uint32_t) __popcll(peers);
+		}
 		__syncthreads();
 		{
 			uint32_t sum = 0;
-			for (int w = 0; w < 4; w++)
+			for (int k = 0; k < RS_SLICES * 4; k++)
 			{
-				const uint32_t c = waveCnt[w][t];
-				waveCnt[w][t] = sum;
+				const uint32_t c = sliceCnt[k][t];
+				sliceCnt[k][t] = sum;
 				sum += c;
 			}
 			tileTotal[t] = sum;
 		}
 		__syncthreads();
-		if (valid)
-		{
-			const uint32_t dst = running[d] + waveCnt[wave][d] + rank;
-			keysOut[dst] = key;
-			valsOut[dst] = val;
-		}
+#pragma unroll
+		for (int sl = 0; sl < RS_SLICES; sl++)
+			if (valid[sl])
+			{
+				const uint32_t dst = running[d[sl]] + sliceCnt[sl * 4 + wave][d[sl]] + rank[sl];
+				keysOut[dst] = key[sl];
+				valsOut[dst] = val[sl];
+			}
 		__syncthreads();
 		running[t] += tileTotal[t];
 	}
