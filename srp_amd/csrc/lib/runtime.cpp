@@ -95,7 +95,10 @@ struct DevBuf
 /* Parameter block of a draw: SrpDrawParams, the draw's assembly record,
  * then (256-aligned) the inline uniform. */
 constexpr size_t kAsmDrawOff = (sizeof(SrpDrawParams) + 63) & ~(size_t) 63;
-constexpr size_t kParamsHead = (kAsmDrawOff + sizeof(AsmDraw) + 255) & ~(size_t) 255;
+/* the draw's eight device counters (assembly totals, error flags, clip
+ * count) ride in the block's padding, so the parameter upload zeroes them */
+constexpr size_t kTotalsOff = (kAsmDrawOff + sizeof(AsmDraw) + 15) & ~(size_t) 15;
+constexpr size_t kParamsHead = (kTotalsOff + 8 * sizeof(uint32_t) + 255) & ~(size_t) 255;
 constexpr size_t kMaxInlineUniform = 16384;
 constexpr size_t kStatWords = (size_t) SRP_STAT_SLOTS * SRP_STAT_STRIDE;
 constexpr size_t kParamsBlock = kParamsHead + kMaxInlineUniform;
@@ -115,7 +118,7 @@ struct Runtime
 	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
 	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
 
-	DevBuf uniform, clip, varyings, genVaryings, counts, totals;
+	DevBuf uniform, clip, varyings, genVaryings, counts;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount;
 	ScanScratch scan;
 	SortScratch sort;
@@ -1089,7 +1092,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.stencil = F->dStencil;
 
 	const size_t usize = vse->uniformSize > fse->uniformSize ? vse->uniformSize : fse->uniformSize;
-	size_t paramsBytes = kAsmDrawOff + sizeof(AsmDraw);
+	size_t paramsBytes = kTotalsOff + 8 * sizeof(uint32_t);
+	memset((uint8_t*) R.hParams + kTotalsOff, 0, 8 * sizeof(uint32_t));
 	if (sp->uniform && usize)
 	{
 		if (usize <= kMaxInlineUniform)
@@ -1159,8 +1163,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 
 	/* Assembly COUNT pass + scans. */
 	uint32_t* counts = R.counts.get<uint32_t>((nIn + 1) * 4 * sizeof(uint32_t));
-	uint32_t* totals = R.totals.get<uint32_t>(8 * sizeof(uint32_t));
-	fillU32(totals, 0u, 8, R.stream);
+	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);   /* zeroed by the upload */
 	AsmArgs A;
 	memset(&A, 0, sizeof(A));
 	A.draws = reinterpret_cast<const AsmDraw*>((const uint8_t*) R.dParams + kAsmDrawOff);
