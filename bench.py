@@ -29,7 +29,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
-from scene_runner import GPU_SO, ORACLE_SO, write_noise_png  # noqa: E402
+from scene_runner import GPU_SO, ORACLE_SO, REF_SO, write_noise_png  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -74,6 +74,57 @@ def read_frame(lib, n):
     return c, d, s
 
 
+def on_one_core(fn):
+    """Run fn() pinned to one host core (restoring the affinity after)."""
+    cores = None
+    try:
+        cores = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(cores)})
+    except (AttributeError, OSError):
+        pass
+    try:
+        return fn()
+    finally:
+        if cores:
+            os.sched_setaffinity(0, cores)
+
+
+def reference_baseline(args, tex, budget_s, port):
+    """The reference renderer compiled in place (oracle/_ref, built from its
+    own sources with its own flags) timed on one host core for ~budget_s;
+    its shaded count per frame is the oracle's (the frames are equal, which
+    is checked).  Returns (baseline dict, frame)."""
+    lib = ctypes.CDLL(REF_SO, mode=ctypes.RTLD_LOCAL)
+    lib.srpBenchMeshSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
+    lib.srpBenchMixSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
+    lib.srpBenchRead.argtypes = [ctypes.c_void_p] * 3
+    setup(lib, args, tex)
+
+    def run():
+        lib.srpBenchFrame()                      # warm-up
+        t0 = time.perf_counter()
+        frames = 0
+        while frames < 2 or (time.perf_counter() - t0 < budget_s and frames < 200):
+            lib.srpBenchFrame()
+            frames += 1
+        return frames, time.perf_counter() - t0
+
+    frames, dt = on_one_core(run)
+    frame = read_frame(lib, args.width * args.height)
+    lib.srpBenchFinish()
+    shaded = port["shaded_per_frame"]
+    return {
+        "value": round(shaded * frames / dt / 1e6, 4),
+        "unit": "shaded Mfrags/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": f"{frames} full frames of the same workload by the compiled reference, {dt:.1f} s, "
+                  f"{dt / frames * 1e3:.1f} ms/frame (shaded count per frame from the oracle, whose frame it equals)",
+        "ms_per_frame": round(dt / frames * 1e3, 3),
+        "port_ms_per_frame": port["ms_per_frame"],
+    }, frame
+
+
 def cpu_baseline(args, tex, budget_s):
     """Oracle frames on one host core until ~budget_s of work (at least 2)."""
     lib = load(ORACLE_SO)
@@ -106,6 +157,7 @@ def cpu_baseline(args, tex, budget_s):
         "kind": "port",
         "sample": f"{frames} full frames of the same workload, {dt:.1f} s, {dt / frames * 1e3:.1f} ms/frame",
         "ms_per_frame": round(dt / frames * 1e3, 3),
+        "shaded_per_frame": st[1] // frames,
     }, frame, st[1] // frames
 
 
@@ -465,10 +517,19 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        cpu, cframe, cpu_shaded = cpu_baseline(args, tex, args.cpu_budget)
-        result["cpu_baseline"] = cpu
+        # the oracle (a C restatement of the reference) checks the frame; the
+        # compiled reference, where it was built, is the timed baseline
+        have_ref = os.path.exists(REF_SO)
+        port_budget = min(args.cpu_budget, 3.0) if have_ref else args.cpu_budget
+        cpu, cframe, cpu_shaded = cpu_baseline(args, tex, port_budget)
         same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(gframe, cframe))
         result["parity_vs_oracle"] = bool(same and cpu_shaded == shaded_per_frame)
+        if have_ref:
+            cpu, rframe = reference_baseline(args, tex, args.cpu_budget, cpu)
+            result["parity_vs_reference"] = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                                                for a, b in zip(gframe, rframe))
+        cpu.pop("shaded_per_frame", None)
+        result["cpu_baseline"] = cpu
         result["speedup_vs_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
 
     if not args.no_batch and args.workload == "mesh":
