@@ -7,6 +7,8 @@ checked against the golden PNGs with the reference harness' tolerance of 1
 (tests/compare.py:8 of kitrofimov/srp)."""
 import os
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -131,6 +133,29 @@ def test_cfg_mix_small(oracle, gpu):
 def test_fuzz(oracle, gpu, seed):
     a, b, sa, sb = run_both(oracle, gpu, "cfg/fuzz", seed=seed)
     assert_same(a, b, sa, sb, f"fuzz seed {seed}")
+
+
+def test_fuzz_sweep_raster_variants(oracle, gpu):
+    """1,000 more randomised scenes, each under one of five rasteriser
+    configurations in turn (batched with and without pixel runs, mask box
+    24 / 8 / 0, and the simple per-pixel kernel)."""
+    lib = gpu.lib
+    lib.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
+    variants = [(1, 1, 24), (1, 0, 24), (1, 1, 0), (1, 1, 8), (0, 1, 24)]
+    bad = []
+    try:
+        for seed in range(200, 1200):
+            v = variants[seed % len(variants)]
+            lib.srpHipSetRasterOptions(*v)
+            a = oracle.run("cfg/fuzz", seed=seed)
+            b = gpu.run("cfg/fuzz", seed=seed)
+            if not (np.array_equal(a.color, b.color) and np.array_equal(a.depth.view(np.uint32), b.depth.view(np.uint32))
+                    and np.array_equal(a.stencil, b.stencil)):
+                bad.append((seed, v))
+    finally:
+        lib.srpHipSetRasterOptions(1, 1, 24)
+    print(f"[parity] fuzz sweep: 1000 seeds, {len(bad)} differing", flush=True)
+    assert not bad, f"differing (seed, options): {bad[:10]}"
 
 
 # ---- against the compiled reference's fixtures (no oracle involved) ----
