@@ -168,6 +168,8 @@ def cpu_baseline(args, tex, budget_s):
         if cores:
             os.sched_setaffinity(0, cores)
     st = stats(lib, "srpOracleGetStats")
+    lib.srpOracleBboxPixels.restype = ctypes.c_uint64
+    bbox = int(lib.srpOracleBboxPixels())
     frame = read_frame(lib, args.width * args.height)
     lib.srpBenchFinish()
     return {
@@ -178,7 +180,36 @@ def cpu_baseline(args, tex, budget_s):
         "sample": f"{frames} full frames of the same workload, {dt:.1f} s, {dt / frames * 1e3:.1f} ms/frame",
         "ms_per_frame": round(dt / frames * 1e3, 3),
         "shaded_per_frame": st[1] // frames,
+        "covered_per_frame": st[0] // frames,
+        "bbox_pixels_per_frame": bbox // frames,
     }, frame, st[1] // frames
+
+
+VALU_FP32_PEAK_TFLOPS = 157.3
+# Reference operation count of the textured mesh frame, per pixel or fragment:
+# 3 barycentric steps per bounding-box pixel (triangle.c:73-111 walks the box),
+# per covered fragment 6 for the depth and 6 for the reciprocal w plus 10 per
+# perspective-interpolated float (two: the uv), and per shaded fragment the
+# fragment shader's 23 (nearest texel: wrap 4, scale 2, convert 2, index 2,
+# unpack 6; shade factor 4; three products).
+VALU_OPS_BBOX, VALU_OPS_COVERED, VALU_OPS_SHADED = 3, 6 + 6 + 2 * 10, 23
+
+
+def valu_figure(bbox, covered, shaded, raster_ms, frame_ms):
+    """Secondary bound: the reference's arithmetic for the frame against the
+    FP32 vector peak (the rasteriser does all of it)."""
+    ops = VALU_OPS_BBOX * bbox + VALU_OPS_COVERED * covered + VALU_OPS_SHADED * shaded
+    achieved = ops / (raster_ms * 1e-3) / 1e12 if raster_ms > 0 else 0.0
+    return {
+        "bound": "valu",
+        "ops_per_frame": int(ops),
+        "bbox_pixels_per_frame": int(bbox),
+        "achieved": round(achieved, 4),
+        "peak": VALU_FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / VALU_FP32_PEAK_TFLOPS, 6),
+        "frame_achieved": round(ops / (frame_ms * 1e-3) / 1e12, 4),
+    }
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -548,7 +579,11 @@ def main():
             cpu, rframe = reference_baseline(args, tex, args.cpu_budget, cpu)
             result["parity_vs_reference"] = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
                                                 for a, b in zip(gframe, rframe))
-        cpu.pop("shaded_per_frame", None)
+        if args.workload == "mesh" and cpu_shaded == shaded_per_frame:
+            result["valu"] = valu_figure(cpu["bbox_pixels_per_frame"], covered_per_frame, shaded_per_frame,
+                                         raster_ms, ms_per_step)
+        for k in ("shaded_per_frame", "covered_per_frame", "bbox_pixels_per_frame"):
+            cpu.pop(k, None)
         result["cpu_baseline"] = cpu
         result["speedup_vs_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
 

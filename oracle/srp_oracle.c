@@ -57,8 +57,12 @@ typedef struct OracleStats
 } OracleStats;
 
 static OracleStats g_stats;
+/* pixels of the triangles' bounding boxes walked by the raster loop
+ * (triangle.c:73-111 visits each once) */
+static uint64_t g_bboxPixels;
 
-void srpOracleResetStats(void) { memset(&g_stats, 0, sizeof(g_stats)); }
+void srpOracleResetStats(void) { memset(&g_stats, 0, sizeof(g_stats)); g_bboxPixels = 0; }
+uint64_t srpOracleBboxPixels(void) { return g_bboxPixels; }
 void srpOracleGetStats(uint64_t out[4])
 {
 	out[0] = g_stats.covered; out[1] = g_stats.shaded;
@@ -865,6 +869,8 @@ static void orcRasterTriangle(OTri* t, const SRPFramebuffer* fb, const SRPShader
 {
 	float row[3] = {t->lambda[0], t->lambda[1], t->lambda[2]};
 	float lam[3] = {row[0], row[1], row[2]};
+	if (t->maxX > t->minX && t->maxY > t->minY)
+		g_bboxPixels += (uint64_t) (t->maxX - t->minX) * (uint64_t) (t->maxY - t->minY);
 	for (size_t y = (size_t) t->minY; y < t->maxY; y++)
 	{
 		for (size_t x = (size_t) t->minX; x < t->maxX; x++)

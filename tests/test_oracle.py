@@ -117,6 +117,11 @@ def test_stencil_counts_shaded(oracle, tex):
     covered, shaded, prims, draws = oracle.stats()
     assert int(f.stencil.astype(np.int64).sum()) == shaded
     assert 0 < shaded <= covered and draws == 1 and prims > 0
+    # every covered fragment lies in its triangle's box; no box exceeds the frame
+    lib = oracle.lib
+    lib.srpOracleBboxPixels.restype = ctypes.c_uint64
+    bbox = int(lib.srpOracleBboxPixels())
+    assert covered <= bbox <= prims * 480 * 270
 
 
 def transcript(path):
