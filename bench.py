@@ -573,6 +573,7 @@ def main():
         have_ref = os.path.exists(REF_SO)
         port_budget = min(args.cpu_budget, 3.0) if have_ref else args.cpu_budget
         cpu, cframe, cpu_shaded = cpu_baseline(args, tex, port_budget)
+        bbox_per_frame = cpu["bbox_pixels_per_frame"]
         same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(gframe, cframe))
         result["parity_vs_oracle"] = bool(same and cpu_shaded == shaded_per_frame)
         if have_ref:
@@ -580,7 +581,7 @@ def main():
             result["parity_vs_reference"] = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
                                                 for a, b in zip(gframe, rframe))
         if args.workload == "mesh" and cpu_shaded == shaded_per_frame:
-            result["valu"] = valu_figure(cpu["bbox_pixels_per_frame"], covered_per_frame, shaded_per_frame,
+            result["valu"] = valu_figure(bbox_per_frame, covered_per_frame, shaded_per_frame,
                                          raster_ms, ms_per_step)
         for k in ("shaded_per_frame", "covered_per_frame", "bbox_pixels_per_frame"):
             cpu.pop(k, None)
