@@ -450,8 +450,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md, not run)", file=sys.stderr)
+        print(f"bench.py: the draw pipeline runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a frame would split by screen tiles, a batch by scene; neither is implemented)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe_mb:
