@@ -397,20 +397,27 @@ __global__ void k_capture_write(CaptureArgs A)
 	A.targ[o * 3 + 2] = (float) ((px >> 8) & 255u) / 255.0f;
 }
 
-/* sum of squared RGB differences, largest difference, pixel count */
+/* Sum of squared RGB differences, largest difference and pixel count, per
+ * block: part[3*block + {0, 1, 2}].  The grid strides over the pixels and
+ * the host adds the blocks' partials (no atomics on one shared address,
+ * which serialised a grid of one block per 256 pixels). */
+constexpr unsigned kCompareBlocks = 1024;
+
 __global__ __launch_bounds__(256) void k_compare(const uint32_t* a, const uint32_t* b, const float* depth, size_t n,
-                                                 unsigned long long* out)
+                                                 unsigned long long* part)
 {
-	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
 	unsigned long long sse = 0, cnt = 0;
 	uint32_t mx = 0;
-	if (i < n && depth[i] < 1.0f)
+	for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
 	{
-		cnt = 1;
+		if (!(depth[i] < 1.0f))
+			continue;
+		cnt++;
+		const uint32_t pa = a[i], pb = b[i];
 #pragma unroll
 		for (int sh = 8; sh <= 24; sh += 8)
 		{
-			const int d = (int) ((a[i] >> sh) & 255u) - (int) ((b[i] >> sh) & 255u);
+			const int d = (int) ((pa >> sh) & 255u) - (int) ((pb >> sh) & 255u);
 			sse += (unsigned long long) (d * d);
 			const uint32_t ad = (uint32_t) (d < 0 ? -d : d);
 			mx = ad > mx ? ad : mx;
@@ -442,9 +449,9 @@ __global__ __launch_bounds__(256) void k_compare(const uint32_t* a, const uint32
 			cnt += sCnt[k];
 			mx = sMx[k] > mx ? sMx[k] : mx;
 		}
-		atomicAdd(&out[0], sse);
-		atomicMax(&out[1], (unsigned long long) mx);
-		atomicAdd(&out[2], cnt);
+		part[3 * blockIdx.x + 0] = sse;
+		part[3 * blockIdx.x + 1] = mx;
+		part[3 * blockIdx.x + 2] = cnt;
 	}
 }
 
@@ -913,14 +920,22 @@ void srpNeuralCompare(const SRPFramebuffer* a, const SRPFramebuffer* b, const SR
 	srp::lib::framebufferDevice(a, &ca, NULL, NULL);
 	srp::lib::framebufferDevice(b, &cb, NULL, NULL);
 	srp::lib::framebufferDevice(mask, NULL, &depth, NULL);
-	unsigned long long* out = devAlloc<unsigned long long>(3);
-	NN_CHECK(hipMemsetAsync(out, 0, 3 * sizeof(unsigned long long), s));
-	hipLaunchKernelGGL(k_compare, dim3((unsigned) ((a->size + 255) / 256)), dim3(256), 0, s, (const uint32_t*) ca,
-	                   (const uint32_t*) cb, (const float*) depth, a->size, out);
-	unsigned long long h[3];
-	NN_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+	const size_t need = (a->size + 255) / 256;
+	const unsigned blocks = need == 0 ? 1u : (need < kCompareBlocks ? (unsigned) need : kCompareBlocks);
+	unsigned long long* part = devAlloc<unsigned long long>(3 * (size_t) blocks);
+	hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(256), 0, s, (const uint32_t*) ca, (const uint32_t*) cb,
+	                   (const float*) depth, a->size, part);
+	unsigned long long hp[3 * kCompareBlocks];
+	NN_CHECK(hipMemcpyAsync(hp, part, 3 * sizeof(unsigned long long) * blocks, hipMemcpyDeviceToHost, s));
 	NN_CHECK(hipStreamSynchronize(s));
-	NN_CHECK(hipFree(out));
+	NN_CHECK(hipFree(part));
+	unsigned long long h[3] = {0, 0, 0};
+	for (unsigned k = 0; k < blocks; k++)
+	{
+		h[0] += hp[3 * k + 0];
+		h[1] = hp[3 * k + 1] > h[1] ? hp[3 * k + 1] : h[1];
+		h[2] += hp[3 * k + 2];
+	}
 	const double mse = h[2] ? (double) h[0] / (3.0 * (double) h[2]) : 0.0;
 	if (psnr) *psnr = mse > 0.0 ? 10.0 * log10(255.0 * 255.0 / mse) : INFINITY;
 	if (maxAbs) *maxAbs = (double) h[1];

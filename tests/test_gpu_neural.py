@@ -4,6 +4,7 @@ forward pass, gradients and the Adam step against the CPU oracle
 G-buffer capture against the oracle's frames; and a small end-to-end run in
 which the network, trained on the GPU from captured fragments, shades
 held-out views of the terrain, with what it gains over no network."""
+import ctypes
 import os
 import sys
 
@@ -193,6 +194,56 @@ def test_capture_matches_oracle_frames(nn, oracle):
     assert got == mask.sum()
     assert np.array_equal(Fg, Fo)
     assert np.array_equal(Tg, To)
+
+
+class _Framebuffer(ctypes.Structure):
+    """SRPFramebuffer of include/srp/framebuffer.h."""
+    _fields_ = [("width", ctypes.c_size_t), ("height", ctypes.c_size_t), ("size", ctypes.c_size_t),
+                ("color", ctypes.POINTER(ctypes.c_uint32)), ("depth", ctypes.POINTER(ctypes.c_float)),
+                ("stencil", ctypes.POINTER(ctypes.c_uint8))]
+
+
+@pytest.mark.parametrize("w,h,covered", [(1920, 1080, 0.6), (7, 3, 0.5), (1, 1, 1.0), (2048, 1031, 0.0)])
+def test_compare_matches_numpy(nn, w, h, covered):
+    """srpNeuralCompare over whole frames (1080p: each thread strides over
+    eight pixels; ragged and empty masks too) equals the PSNR, largest
+    difference and pixel count numpy computes over the covered pixels."""
+    L = nn.lib
+    L.srpNewFramebuffer.restype = ctypes.POINTER(_Framebuffer)
+    L.srpNewFramebuffer.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+    L.srpFreeFramebuffer.argtypes = [ctypes.POINTER(_Framebuffer)]
+    rng = np.random.default_rng(w * 7919 + h)
+    fbs = [L.srpNewFramebuffer(w, h) for _ in range(3)]
+    try:
+        n = w * h
+        arrays = []
+        for fb in fbs:
+            c = np.ctypeslib.as_array(fb.contents.color, (n,))
+            d = np.ctypeslib.as_array(fb.contents.depth, (n,))
+            c[:] = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+            d[:] = np.where(rng.uniform(size=n) < covered, rng.uniform(0, 0.999, n), 1.0).astype(np.float32)
+            arrays.append((c.copy(), d.copy()))
+        # the second frame differs from the first by a little on most pixels
+        ca = arrays[0][0]
+        cb = (ca.astype(np.int64) ^ (rng.integers(0, 4, n) << (8 * rng.integers(1, 4, n)))).astype(np.uint32)
+        np.ctypeslib.as_array(fbs[1].contents.color, (n,))[:] = cb
+        ps, mx, npx = nn.compare(ctypes.cast(fbs[0], ctypes.c_void_p), ctypes.cast(fbs[1], ctypes.c_void_p),
+                                 ctypes.cast(fbs[2], ctypes.c_void_p))
+        m = arrays[2][1] < 1
+
+        def rgb(c):
+            return np.stack([(c >> 24) & 255, (c >> 16) & 255, (c >> 8) & 255], -1).astype(np.uint8)
+
+        assert npx == int(m.sum())
+        if npx:
+            pn, mn = no.psnr(rgb(ca)[m], rgb(cb)[m])
+            assert mx == mn
+            assert ps == pytest.approx(pn, abs=1e-9) if np.isfinite(pn) else ps == pn
+        else:
+            assert mx == 0.0
+    finally:
+        for fb in fbs:
+            L.srpFreeFramebuffer(fb)
 
 
 def test_neural_shader_end_to_end(nn):
