@@ -925,8 +925,8 @@ void srpNeuralCompare(const SRPFramebuffer* a, const SRPFramebuffer* b, const SR
 	unsigned long long* part = devAlloc<unsigned long long>(3 * (size_t) blocks);
 	hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(256), 0, s, (const uint32_t*) ca, (const uint32_t*) cb,
 	                   (const float*) depth, a->size, part);
-	unsigned long long hp[3 * kCompareBlocks];
-	NN_CHECK(hipMemcpyAsync(hp, part, 3 * sizeof(unsigned long long) * blocks, hipMemcpyDeviceToHost, s));
+	std::vector<unsigned long long> hp(3 * (size_t) blocks);
+	NN_CHECK(hipMemcpyAsync(hp.data(), part, 3 * sizeof(unsigned long long) * blocks, hipMemcpyDeviceToHost, s));
 	NN_CHECK(hipStreamSynchronize(s));
 	NN_CHECK(hipFree(part));
 	unsigned long long h[3] = {0, 0, 0};
