@@ -1,0 +1,67 @@
+"""Error behaviour of the learned-shader C ABI (include/srp/srp_neural.h),
+on the CPU: invalid arguments are rejected before any device work and
+reported through the context's message callback (the reference's error
+channel, utils/message_callback.c:18-35), naming the API function."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from scene_runner import REPO
+
+LIB_SO = os.path.join(REPO, "srp_amd", "libsrp_amd.so")
+
+CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p)
+
+
+class _Callback(ctypes.Structure):
+    _fields_ = [("func", CALLBACK), ("userParameter", ctypes.c_void_p)]
+
+
+@pytest.fixture(scope="module")
+def api():
+    if not os.path.exists(LIB_SO):
+        pytest.skip("libsrp_amd.so is not built")
+    lib = ctypes.CDLL(LIB_SO, mode=ctypes.RTLD_LOCAL)
+    got = []
+
+    def on_message(kind, severity, fn, text, user):
+        got.append((fn.decode(), text.decode()))
+
+    cb = CALLBACK(on_message)
+    lib.srpSetMessageCallback.argtypes = [_Callback]
+    lib.srpSetMessageCallback(_Callback(cb, None))
+    yield lib, got
+    lib.srpSetMessageCallback(_Callback(CALLBACK(), None))
+
+
+def test_dataset_capacity_must_be_positive(api):
+    lib, got = api
+    got.clear()
+    lib.srpNewNeuralDataset.restype = ctypes.c_void_p
+    lib.srpNewNeuralDataset.argtypes = [ctypes.c_size_t]
+    assert lib.srpNewNeuralDataset(0) is None
+    assert got == [("srpNewNeuralDataset", "Capacity must be positive")]
+
+
+@pytest.mark.parametrize("bad", [0.0, -1.0, 1024.0, float("nan"), float("inf")])
+def test_encoding_step_out_of_range(api, bad):
+    lib, got = api
+    got.clear()
+    lib.srpNeuralNetSetEncoding.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    steps = np.full(8, 0.5, np.float32)
+    steps[5] = bad
+    lib.srpNeuralNetSetEncoding(None, steps.ctypes.data)   # rejected before the net is touched
+    assert len(got) == 1 and got[0][0] == "srpNeuralNetSetEncoding"
+    assert "feature 5 is not in (0, 1024)" in got[0][1]
+
+
+def test_compare_needs_framebuffers_of_one_size(api):
+    lib, got = api
+    got.clear()
+    lib.srpNeuralCompare.argtypes = [ctypes.c_void_p] * 6
+    ps, mx, n = ctypes.c_double(-1), ctypes.c_double(-1), ctypes.c_uint64(7)
+    lib.srpNeuralCompare(None, None, None, ctypes.addressof(ps), ctypes.addressof(mx), ctypes.addressof(n))
+    assert got == [("srpNeuralCompare", "Framebuffers must exist and have one size")]
+    assert ps.value == -1 and mx.value == -1 and n.value == 7   # outputs untouched
