@@ -1,5 +1,5 @@
-"""Error behaviour of the learned-shader C ABI (include/srp/srp_neural.h),
-on the CPU: invalid arguments are rejected before any device work and
+"""Error behaviour of the learned-shader and batch C ABI
+(include/srp/srp_neural.h, include/srp/srp_hip.h), on the CPU: invalid arguments are rejected before any device work and
 reported through the context's message callback (the reference's error
 channel, utils/message_callback.c:18-35), naming the API function."""
 import ctypes
@@ -55,6 +55,15 @@ def test_encoding_step_out_of_range(api, bad):
     lib.srpNeuralNetSetEncoding(None, steps.ctypes.data)   # rejected before the net is touched
     assert len(got) == 1 and got[0][0] == "srpNeuralNetSetEncoding"
     assert "feature 5 is not in (0, 1024)" in got[0][1]
+
+
+def test_batch_needs_draws(api):
+    lib, got = api
+    got.clear()
+    lib.srpNewBatch.restype = ctypes.c_void_p
+    lib.srpNewBatch.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    assert lib.srpNewBatch(None, 0) is None
+    assert got == [("srpNewBatch", "No draws")]
 
 
 def test_compare_needs_framebuffers_of_one_size(api):

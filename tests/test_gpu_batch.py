@@ -80,6 +80,35 @@ def test_tunings_give_identical_frames(gpu_batch):
         gb.finish()
 
 
+def test_set_tuning_rejects_out_of_range(gpu_batch):
+    """srpBatchSetTuning reports values outside waves/CU 1..32 or mask box
+    0..24 through the message callback and keeps the batch as it was."""
+    import ctypes
+    gb = gpu_batch
+    gb.setup(8, 0.05, 13, batched=True)
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p)
+
+    class Callback(ctypes.Structure):
+        _fields_ = [("func", CB), ("userParameter", ctypes.c_void_p)]
+
+    got = []
+    cb = CB(lambda kind, sev, fn, text, user: got.append(fn.decode()))
+    gb.lib.srpSetMessageCallback.argtypes = [Callback]
+    try:
+        gb.set_tuning(8, 16, 1, 1)
+        gb.frame(True)
+        base = gb.read_all()
+        gb.lib.srpSetMessageCallback(Callback(cb, None))
+        for waves, box in [(0, 16), (33, 16), (8, 25), (8, -1)]:
+            gb.set_tuning(waves, box, 1, 1)
+        assert got == ["srpBatchSetTuning"] * 4
+        gb.frame(True)
+        assert frames_equal(gb.read_all(), base) == []
+    finally:
+        gb.lib.srpSetMessageCallback(Callback(CB(), None))
+        gb.finish()
+
+
 def test_autotuner_picks_a_candidate_and_keeps_results(gpu_batch):
     gb = gpu_batch
     gb.setup(16, 0.05, 9, batched=True)
