@@ -57,6 +57,28 @@ def test_encoding_step_out_of_range(api, bad):
     assert "feature 5 is not in (0, 1024)" in got[0][1]
 
 
+@pytest.mark.parametrize("call, fn, text", [
+    (lambda L: L.srpNeuralDatasetStage(None, None, 0), "srpNeuralDatasetStage", "Invalid dataset, framebuffer or slot (0)"),
+    (lambda L: L.srpNeuralDatasetAppend(None, None, ctypes.c_float(1.0), ctypes.c_uint64(1)),
+     "srpNeuralDatasetAppend", "Nothing staged for a framebuffer of this size"),
+    (lambda L: L.srpNeuralDatasetRead(None, ctypes.c_size_t(0), ctypes.c_size_t(1), None, None),
+     "srpNeuralDatasetRead", "Range out of bounds"),
+    (lambda L: L.srpNeuralDatasetWrite(None, None, None, ctypes.c_size_t(1)),
+     "srpNeuralDatasetWrite", "1 samples exceed the capacity"),
+    (lambda L: L.srpNeuralNetTrain(None, None, 10, 8192, ctypes.c_float(1e-3), ctypes.c_float(1e-4),
+                                   ctypes.c_uint64(1), None), "srpNeuralNetTrain", "Nothing to train on"),
+    (lambda L: L.srpNeuralNetGradients(None, None, ctypes.c_size_t(256), None, None),
+     "srpNeuralNetGradients", "Count exceeds the dataset"),
+])
+def test_missing_objects_are_reported(api, call, fn, text):
+    """Every dataset and training call given no object reports it and
+    returns before any device work (nothing here has a GPU)."""
+    lib, got = api
+    got.clear()
+    call(lib)
+    assert got == [(fn, text)]
+
+
 def test_batch_needs_draws(api):
     lib, got = api
     got.clear()

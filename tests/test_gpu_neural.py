@@ -196,6 +196,37 @@ def test_capture_matches_oracle_frames(nn, oracle):
     assert np.array_equal(Tg, To)
 
 
+@pytest.mark.parametrize("batch", [0, 1000, 65536 + 1024])
+def test_train_rejects_invalid_batch(nn, batch):
+    """A batch that is not a positive multiple of 1,024 samples, or above
+    65,536, is reported through the message callback; the parameters stay
+    as they were."""
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p)
+
+    class Callback(ctypes.Structure):
+        _fields_ = [("func", CB), ("userParameter", ctypes.c_void_p)]
+
+    got = []
+    cb = CB(lambda kind, sev, fn, text, user: got.append((fn.decode(), text.decode())))
+    nn.lib.srpSetMessageCallback.argtypes = [Callback]
+    F, T = synth(4096, 77)
+    ds = nn.new_ds(4096)
+    net = nn.new_net(5)
+    try:
+        nn.ds_write(ds, F, T)
+        before = nn.get_params(net)
+        nn.lib.srpSetMessageCallback(Callback(cb, None))
+        nn._train(net, ds, 3, batch, 1e-3, 1e-4, 1, None)
+        nn.lib.srpSetMessageCallback(Callback(CB(), None))
+        assert len(got) == 1 and got[0][0] == "srpNeuralNetTrain"
+        assert f"Batch of {batch} samples" in got[0][1]
+        assert np.array_equal(nn.get_params(net), before)
+    finally:
+        nn.lib.srpSetMessageCallback(Callback(CB(), None))
+        nn.free_net(net)
+        nn.free_ds(ds)
+
+
 class _Framebuffer(ctypes.Structure):
     """SRPFramebuffer of include/srp/framebuffer.h."""
     _fields_ = [("width", ctypes.c_size_t), ("height", ctypes.c_size_t), ("size", ctypes.c_size_t),
