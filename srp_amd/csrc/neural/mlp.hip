@@ -774,17 +774,35 @@ void srpFreeNeuralNet(SRPNeuralNet* N)
 
 void srpNeuralNetGetParams(const SRPNeuralNet* N, float* params)
 {
+	if (!N || !params)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid network or parameter array");
+		return;
+	}
 	hipStream_t s = libStream();
 	NN_CHECK(hipMemcpyAsync(params, N->M.w, SRP_NN_PARAMS * sizeof(float), hipMemcpyDeviceToHost, s));
 	NN_CHECK(hipStreamSynchronize(s));
 }
 
-void srpNeuralNetSetParams(SRPNeuralNet* N, const float* params) { uploadParams(N, params); }
+void srpNeuralNetSetParams(SRPNeuralNet* N, const float* params)
+{
+	if (!N || !params)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid network or parameter array");
+		return;
+	}
+	uploadParams(N, params);
+}
 
 const void* srpNeuralNetDevice(const SRPNeuralNet* N) { return N ? (const void*) N->view : NULL; }
 
 void srpNeuralNetSetEncoding(SRPNeuralNet* N, const float* step)
 {
+	if (!step)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid step array");
+		return;
+	}
 	for (int d = 0; d < SRP_NN_IN; d++)
 		if (!(step[d] > 0.0f && step[d] < 1024.0f))
 		{
@@ -792,6 +810,11 @@ void srpNeuralNetSetEncoding(SRPNeuralNet* N, const float* step)
 			           "frequency step %g of feature %d is not in (0, 1024)", (double) step[d], d);
 			return;
 		}
+	if (!N)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid network");
+		return;
+	}
 	hipStream_t s = libStream();
 	NN_CHECK(hipStreamSynchronize(s));
 	for (int d = 0; d < SRP_NN_IN; d++)
@@ -801,6 +824,11 @@ void srpNeuralNetSetEncoding(SRPNeuralNet* N, const float* step)
 
 void srpNeuralNetGetEncoding(const SRPNeuralNet* N, float* step)
 {
+	if (!N || !step)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid network or step array");
+		return;
+	}
 	for (int d = 0; d < SRP_NN_IN; d++)
 		step[d] = N->host.step[d];
 }

@@ -69,6 +69,14 @@ def test_encoding_step_out_of_range(api, bad):
                                    ctypes.c_uint64(1), None), "srpNeuralNetTrain", "Nothing to train on"),
     (lambda L: L.srpNeuralNetGradients(None, None, ctypes.c_size_t(256), None, None),
      "srpNeuralNetGradients", "Count exceeds the dataset"),
+    (lambda L: L.srpNeuralNetInfer(None, None, ctypes.c_size_t(1), None, 0),
+     "srpNeuralNetInfer", "Count exceeds the dataset"),
+    (lambda L: L.srpNeuralNetGetParams(None, None), "srpNeuralNetGetParams", "Invalid network or parameter array"),
+    (lambda L: L.srpNeuralNetSetParams(None, None), "srpNeuralNetSetParams", "Invalid network or parameter array"),
+    (lambda L: L.srpNeuralNetGetEncoding(None, None), "srpNeuralNetGetEncoding", "Invalid network or step array"),
+    (lambda L: L.srpNeuralNetSetEncoding(None, None), "srpNeuralNetSetEncoding", "Invalid step array"),
+    (lambda L: L.srpNeuralNetSetEncoding(None, np.full(8, 0.5, np.float32).ctypes.data),
+     "srpNeuralNetSetEncoding", "Invalid network"),
 ])
 def test_missing_objects_are_reported(api, call, fn, text):
     """Every dataset and training call given no object reports it and
