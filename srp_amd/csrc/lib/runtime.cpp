@@ -638,6 +638,11 @@ void srpFramebufferClear(const SRPFramebuffer* fb)
 
 void srpFramebufferSetDeferred(const SRPFramebuffer* fb, bool deferred)
 {
+	if (!fb)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
+		return;
+	}
 	FbImpl* F = fbImpl(fb);
 	if (deferred && !F->deferred)
 		fbUpload(F);
@@ -646,16 +651,34 @@ void srpFramebufferSetDeferred(const SRPFramebuffer* fb, bool deferred)
 	F->deferred = deferred;
 }
 
-void srpFramebufferDownload(const SRPFramebuffer* fb) { fbDownload(fbImpl(fb)); }
+void srpFramebufferDownload(const SRPFramebuffer* fb)
+{
+	if (!fb)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
+		return;
+	}
+	fbDownload(fbImpl(fb));
+}
 
 void srpFramebufferUpload(const SRPFramebuffer* fb)
 {
+	if (!fb)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
+		return;
+	}
 	fbUpload(fbImpl(fb));
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
 }
 
 void srpFramebufferFillDepth(const SRPFramebuffer* fb, float v)
 {
+	if (!fb)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
+		return;
+	}
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 	{
@@ -668,6 +691,11 @@ void srpFramebufferFillDepth(const SRPFramebuffer* fb, float v)
 
 void srpFramebufferFillStencil(const SRPFramebuffer* fb, uint8_t v)
 {
+	if (!fb)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
+		return;
+	}
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 	{

@@ -1,5 +1,5 @@
-"""Error behaviour of the learned-shader and batch C ABI
-(include/srp/srp_neural.h, include/srp/srp_hip.h), on the CPU: invalid arguments are rejected before any device work and
+"""Error behaviour of the learned-shader, batch and framebuffer-extension
+C ABI (include/srp/srp_neural.h, include/srp/srp_hip.h), on the CPU: invalid arguments are rejected before any device work and
 reported through the context's message callback (the reference's error
 channel, utils/message_callback.c:18-35), naming the API function."""
 import ctypes
@@ -85,6 +85,20 @@ def test_missing_objects_are_reported(api, call, fn, text):
     got.clear()
     call(lib)
     assert got == [(fn, text)]
+
+
+@pytest.mark.parametrize("fn, args", [
+    ("srpFramebufferSetDeferred", (None, True)),
+    ("srpFramebufferDownload", (None,)),
+    ("srpFramebufferUpload", (None,)),
+    ("srpFramebufferFillDepth", (None, ctypes.c_float(1.0))),
+    ("srpFramebufferFillStencil", (None, ctypes.c_uint8(0))),
+])
+def test_framebuffer_extensions_need_a_framebuffer(api, fn, args):
+    lib, got = api
+    got.clear()
+    getattr(lib, fn)(*args)
+    assert got == [(fn, "Invalid framebuffer")]
 
 
 def test_batch_needs_draws(api):
