@@ -38,6 +38,8 @@ def load(path):
     lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
     lib.srpBenchMeshSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     lib.srpBenchMixSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
+    lib.srpBenchTrisSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
+    lib.srpBenchTerrainSetup.argtypes = [ctypes.c_int] * 6
     lib.srpBenchRead.argtypes = [ctypes.c_void_p] * 3
     return lib
 
@@ -48,22 +50,195 @@ def stats(lib, fn):
     return list(a)
 
 
-def setup(lib, args, tex):
-    if args.workload == "mesh":
-        rc = lib.srpBenchMeshSetup(args.width, args.height, args.rings, args.segs, tex.encode())
+# The workloads, each a dict of its sizes:
+#   mesh     indexed textured UV sphere (the headline)
+#   tris     random clip-space triangles, 512x512 (the plain-method workload)
+#   mix      clipped triangles + lines + points, 1080p
+#   terrain  1,000,000-triangle heightfield with the procedural material, 1080p
+def workload(args, kind, scale=1.0):
+    if kind == "mesh":
+        return dict(kind=kind, w=args.width, h=args.height, rings=args.rings, segs=args.segs)
+    if kind == "tris":
+        return dict(kind=kind, w=512, h=512, tris=max(1, int(round(2000 * scale))), seed=0x5EED0001)
+    if kind == "mix":
+        return dict(kind=kind, w=args.width, h=args.height, tris=max(1, int(round(args.tris * scale))),
+                    lines=max(1, int(round(args.lines * scale))), points=max(4, int(round(args.points * scale))),
+                    seed=0x5EED0003)
+    return dict(kind=kind, w=args.width, h=args.height, gx=args.grid_x, gz=args.grid_z, pose=256)
+
+
+def describe(wl):
+    k = wl["kind"]
+    if k == "mesh":
+        return (f"indexed textured UV sphere {wl['rings']}x{wl['segs']} segs, {2 * wl['rings'] * wl['segs']:,} tris, "
+                f"{wl['w']}x{wl['h']}, scissor 75%, stencil INCR, depth LESS")
+    if k == "tris":
+        return f"{wl['tris']:,} random clip-space triangles, {wl['w']}x{wl['h']}, depth LESS, no culling"
+    if k == "mix":
+        return (f"{wl['tris']:,} tris + {wl['lines']:,} lines + {wl['points']:,} points, {wl['w']}x{wl['h']}, "
+                f"LEQUAL, cull back")
+    return (f"{wl['gx']}x{wl['gz']}-cell terrain ({2 * wl['gx'] * wl['gz']:,} tris), procedural fBm + Blinn-Phong "
+            f"material, pose {wl['pose']}, {wl['w']}x{wl['h']}, depth LESS, cull back")
+
+
+def setup(lib, wl, tex):
+    k = wl["kind"]
+    if k == "mesh":
+        rc = lib.srpBenchMeshSetup(wl["w"], wl["h"], wl["rings"], wl["segs"], tex.encode())
+    elif k == "tris":
+        rc = lib.srpBenchTrisSetup(wl["w"], wl["h"], wl["tris"], wl["seed"])
+    elif k == "mix":
+        rc = lib.srpBenchMixSetup(wl["w"], wl["h"], wl["tris"], wl["lines"], wl["points"], wl["seed"])
     else:
-        rc = lib.srpBenchMixSetup(args.width, args.height, args.tris, args.lines, args.points, 0x5EED0003)
+        rc = lib.srpBenchTerrainSetup(wl["w"], wl["h"], wl["gx"], wl["gz"], wl["pose"], 0)
     if rc != 0:
         raise RuntimeError(f"bench setup failed: {rc}")
 
 
-def geometry_bytes(args):
+def input_triangles(wl):
+    k = wl["kind"]
+    return {"mesh": lambda: 2 * wl["rings"] * wl["segs"], "tris": lambda: wl["tris"], "mix": lambda: wl["tris"],
+            "terrain": lambda: 2 * wl["gx"] * wl["gz"]}[k]()
+
+
+def geometry_bytes(wl):
     """Vertex + index bytes one frame reads (each unique vertex once)."""
-    if args.workload == "mesh":
-        nv = (args.rings + 1) * (args.segs + 1)
-        return nv * 20 + args.rings * args.segs * 6 * 4
-    # primitive mix: non-indexed {clip vec4, st vec2, level, tag} = 32 B per vertex
-    return (3 * args.tris + 2 * args.lines + args.points) * 32
+    k = wl["kind"]
+    if k == "mesh":
+        nv = (wl["rings"] + 1) * (wl["segs"] + 1)
+        return nv * 20 + wl["rings"] * wl["segs"] * 6 * 4
+    if k == "tris":
+        return 3 * wl["tris"] * 24                      # {pos vec3, colour vec3}
+    if k == "mix":
+        # non-indexed {clip vec4, st vec2, level, tag} = 32 B per vertex
+        return (3 * wl["tris"] + 2 * wl["lines"] + wl["points"]) * 32
+    nv = (wl["gx"] + 1) * (wl["gz"] + 1)
+    return nv * 32 + 2 * wl["gx"] * wl["gz"] * 3 * 4    # {pos, normal, uv} + u32 indices
+
+
+def raster_bytes(wl, shaded_per_frame):
+    """Algorithmic bytes of the triangle rasteriser per frame: colour, depth
+    and stencil read and written once (W*H*9*2), plus one RGBA8 texel per
+    shaded fragment for the textured mesh."""
+    return wl["w"] * wl["h"] * 9 * 2 + shaded_per_frame * (4 if wl["kind"] == "mesh" else 0)
+
+
+def gpu_frames(wl, tex, steps, warmup):
+    """GPU frames of a workload (inputs resident, deferred framebuffer):
+    wall time per frame, per-kernel HIP-event times, counters, last frame."""
+    gpu = load(GPU_SO)
+    setup(gpu, wl, tex)
+    for _ in range(warmup):
+        gpu.srpBenchFrame()
+    gpu.srpHipSynchronize()
+    gpu.srpHipResetStats()
+    gpu.srpHipProfileEnable(True)
+    gpu.srpHipSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu.srpBenchFrame()
+    gpu.srpHipSynchronize()
+    dt = time.perf_counter() - t0
+    ms = (ctypes.c_double * 4)()
+    launches = (ctypes.c_uint64 * 4)()
+    gpu.srpHipProfileRead(ms, launches)
+    gpu.srpHipProfileEnable(False)
+    st = stats(gpu, "srpHipGetStats")
+    frame = read_frame(gpu, wl["w"] * wl["h"])
+    gpu.srpBenchFinish()
+    return {
+        "ms_per_frame": dt / steps * 1e3,
+        "shaded_per_frame": st[1] // steps,
+        "covered_tested_per_frame": st[0] // steps,
+        "primitives_per_frame": st[2] // steps,
+        "draws_per_frame": st[3] // steps,
+        "raster_ms": ms[0] / max(launches[0], 1),
+        "draw_ms": ms[1] / max(launches[1], 1),
+        "frag_ms": ms[2] / max(launches[2], 1) if launches[2] else None,
+        "front_ms": ms[3] / max(launches[3], 1),
+        "raster_launches_per_frame": launches[0] / steps,
+    }, frame
+
+
+def reference_frames(wl, tex, budget_s, min_frames, max_frames):
+    """The reference renderer compiled in place (oracle/_ref), or the oracle
+    where it was not built, timed on one host core: frames until budget_s
+    (at least min_frames, at most max_frames) after one warm-up frame."""
+    path = REF_SO if os.path.exists(REF_SO) else ORACLE_SO
+    lib = load(path)
+    setup(lib, wl, tex)
+
+    def run():
+        lib.srpBenchFrame()                      # warm-up
+        t0 = time.perf_counter()
+        times = []
+        while len(times) < min_frames or (time.perf_counter() - t0 < budget_s and len(times) < max_frames):
+            t = time.perf_counter()
+            lib.srpBenchFrame()
+            times.append(time.perf_counter() - t)
+        return times
+
+    times = on_one_core(run)
+    frame = read_frame(lib, wl["w"] * wl["h"])
+    lib.srpBenchFinish()
+    return ("reference" if path == REF_SO else "port"), times, frame
+
+
+def frames_equal(a, b):
+    return all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(a, b))
+
+
+def workload_leg(args, tex, kind, steps, warmup, cpu_scale, cpu_budget, cpu_min, cpu_max):
+    """One workload at full size on the GPU, beside the CPU reference on a
+    bounded sample of it (the same generator at cpu_scale of its primitive
+    counts, or the full frame when cpu_scale is 1).  The CPU's shaded count
+    is the GPU's on the same sample, whose frame must equal the reference's
+    (checked)."""
+    wl = workload(args, kind)
+    g, gframe = gpu_frames(wl, tex, steps, warmup)
+    out = {
+        "workload": describe(wl),
+        "frame_ms": round(g["ms_per_frame"], 4),
+        "shaded_per_frame": int(g["shaded_per_frame"]),
+        "shaded_mfrags_per_s": round(g["shaded_per_frame"] / g["ms_per_frame"] * 1e-3, 3),
+        "triangles_per_s": round(input_triangles(wl) / (g["ms_per_frame"] * 1e-3), 1),
+        "primitives_per_frame": int(g["primitives_per_frame"]),
+        "kernel_ms": {"raster_tris": round(g["raster_ms"], 4), "fragment_lists": round(g["frag_ms"], 4)
+                      if g["frag_ms"] is not None else None, "vertex_and_count_pass": round(g["front_ms"], 4)},
+    }
+    rb = raster_bytes(wl, g["shaded_per_frame"])
+    if g["raster_ms"] > 0:
+        ach = rb / (g["raster_ms"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "srp_raster_batched", "achieved": round(ach, 2),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6),
+                           "algorithmic_bytes": int(rb),
+                           "frame_bytes": int(rb + geometry_bytes(wl)),
+                           "frame_frac": round((rb + geometry_bytes(wl)) / (g["ms_per_frame"] * 1e-3) / 1e9
+                                               / HBM_PEAK_GBS, 6)}
+    if cpu_budget <= 0:
+        return out
+    swl = workload(args, kind, cpu_scale)
+    kind_cpu, times, cframe = reference_frames(swl, tex, cpu_budget, cpu_min, cpu_max)
+    if cpu_scale == 1.0:
+        sframe, sshaded = gframe, g["shaded_per_frame"]
+    else:
+        s, sframe = gpu_frames(swl, tex, 1, 0)
+        sshaded = s["shaded_per_frame"]
+    med = float(np.median(times))
+    model, ncpu = host_cpu()
+    out["cpu_baseline"] = {
+        "value": round(sshaded / med / 1e6, 4), "unit": "shaded Mfrags/s", "cores": 1, "kind": kind_cpu,
+        "sample": (f"{len(times)} frame(s) of " + ("the same frame" if cpu_scale == 1.0 else
+                   f"the same generator at {cpu_scale:g} of its primitive counts ({describe(swl)})") +
+                   f" after one warm-up frame; median {med * 1e3:.1f} ms/frame; shaded count of that frame "
+                   f"{sshaded:,} (the GPU's, whose frame equals the reference's)"),
+        "ms_per_frame_median": round(med * 1e3, 3),
+        "host_cpu": model, "host_cpus_visible": ncpu,
+    }
+    out["parity_sample_vs_" + kind_cpu] = frames_equal(sframe, cframe)
+    out["speedup_vs_cpu"] = round(out["shaded_mfrags_per_s"] / out["cpu_baseline"]["value"], 1) \
+        if out["cpu_baseline"]["value"] else None
+    return out
 
 
 def read_frame(lib, n):
@@ -103,16 +278,13 @@ def on_one_core(fn):
             os.sched_setaffinity(0, cores)
 
 
-def reference_baseline(args, tex, budget_s, port):
+def reference_baseline(args, wl, tex, budget_s, port):
     """The reference renderer compiled in place (oracle/_ref, built from its
     own sources with its own flags) timed on one host core for ~budget_s;
     its shaded count per frame is the oracle's (the frames are equal, which
     is checked).  Returns (baseline dict, frame)."""
-    lib = ctypes.CDLL(REF_SO, mode=ctypes.RTLD_LOCAL)
-    lib.srpBenchMeshSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
-    lib.srpBenchMixSetup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
-    lib.srpBenchRead.argtypes = [ctypes.c_void_p] * 3
-    setup(lib, args, tex)
+    lib = load(REF_SO)
+    setup(lib, wl, tex)
 
     def run():
         lib.srpBenchFrame()                      # warm-up
@@ -126,7 +298,7 @@ def reference_baseline(args, tex, budget_s, port):
 
     frames, dt, median = on_one_core(run)
     model, ncpu = host_cpu()
-    frame = read_frame(lib, args.width * args.height)
+    frame = read_frame(lib, wl["w"] * wl["h"])
     lib.srpBenchFinish()
     shaded = port["shaded_per_frame"]
     return {
@@ -145,10 +317,10 @@ def reference_baseline(args, tex, budget_s, port):
     }, frame
 
 
-def cpu_baseline(args, tex, budget_s):
+def cpu_baseline(args, wl, tex, budget_s):
     """Oracle frames on one host core until ~budget_s of work (at least 2)."""
     lib = load(ORACLE_SO)
-    setup(lib, args, tex)
+    setup(lib, wl, tex)
     cores = None
     try:
         cores = os.sched_getaffinity(0)
@@ -170,7 +342,7 @@ def cpu_baseline(args, tex, budget_s):
     st = stats(lib, "srpOracleGetStats")
     lib.srpOracleBboxPixels.restype = ctypes.c_uint64
     bbox = int(lib.srpOracleBboxPixels())
-    frame = read_frame(lib, args.width * args.height)
+    frame = read_frame(lib, wl["w"] * wl["h"])
     lib.srpBenchFinish()
     return {
         "value": round(st[1] / dt / 1e6, 4),
@@ -415,7 +587,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["mesh", "mix"], default="mesh")
+    ap.add_argument("--workload", choices=["mesh", "mix", "tris", "terrain"], default="mesh")
+    ap.add_argument("--no-legs", action="store_true", help="skip the primitive-mix, random-triangle and terrain legs")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--rings", type=int, default=256)
@@ -466,6 +639,7 @@ def main():
 
     tmp = tempfile.mkdtemp(prefix="srp_bench_")
     tex = write_noise_png(os.path.join(tmp, "noise1024.png"), 1024, 0x5EED0002)
+    wl = workload(args, args.workload)
 
     gpu = load(GPU_SO)
     if args.tile:
@@ -473,56 +647,32 @@ def main():
     if args.mask_box >= 0:
         gpu.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
         gpu.srpHipSetRasterOptions(-1, -1, args.mask_box)
-    setup(gpu, args, tex)
-    for _ in range(args.warmup):
-        gpu.srpBenchFrame()
-    gpu.srpHipSynchronize()
-
-    gpu.srpHipResetStats()
-    gpu.srpHipProfileEnable(True)
-    gpu.srpHipSynchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        gpu.srpBenchFrame()
-    gpu.srpHipSynchronize()
-    t1 = time.perf_counter()
-    ms = (ctypes.c_double * 4)()
-    launches = (ctypes.c_uint64 * 4)()
-    gpu.srpHipProfileRead(ms, launches)
-    gpu.srpHipProfileEnable(False)
-    st = stats(gpu, "srpHipGetStats")
-    n_px = args.width * args.height
-    gframe = read_frame(gpu, n_px)
-    gpu.srpBenchFinish()
-
-    dt = t1 - t0
+    tile = int(gpu.srpHipGetTileSize())
+    g, gframe = gpu_frames(wl, tex, args.steps, args.warmup)
     steps = args.steps
-    shaded_per_frame = st[1] // steps
-    covered_per_frame = st[0] // steps
-    prims_per_frame = st[2] // steps
-    draws_per_frame = st[3] // steps
-    value = st[1] / dt / 1e6
-    ms_per_step = dt / steps * 1e3
-    in_tris = (args.rings * args.segs * 2) if args.workload == "mesh" else args.tris
+    shaded_per_frame = g["shaded_per_frame"]
+    ms_per_step = g["ms_per_frame"]
+    value = shaded_per_frame / (ms_per_step * 1e-3) / 1e6
+    raster_ms = g["raster_ms"]
 
     # Dominant kernel: the triangle rasteriser.  Algorithmic bytes per launch
     # (the per-frame figure of the workload definition): colour + depth +
     # stencil of the frame read and written once (W*H*9*2) plus one RGBA8
     # texel per shaded fragment (mesh workload).  The whole frame adds the
     # vertex and index data: B_frame = VB + IB + W*H*9*2 + shaded*T_tex.
-    raster_ms = ms[0] / max(launches[0], 1)
-    tex_bytes = 4 if args.workload == "mesh" else 0
-    raster_bytes = n_px * 9 * 2 + shaded_per_frame * tex_bytes
-    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9 if raster_ms > 0 else 0.0
-    frame_bytes = raster_bytes + geometry_bytes(args)
+    rbytes = raster_bytes(wl, shaded_per_frame)
+    achieved = rbytes / (raster_ms * 1e-3) / 1e9 if raster_ms > 0 else 0.0
+    frame_bytes = rbytes + geometry_bytes(wl)
     traffic = None
     tpath = os.path.join(REPO, "profiles", f"raster_{args.workload}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    names = {"mesh": "indexed textured mesh", "mix": "primitive mix", "tris": "random triangles, 512x512",
+             "terrain": "terrain, procedural material"}
 
     result = {
-        "metric": "shaded Mfrags/s @1080p (indexed textured mesh)" if args.workload == "mesh" else "shaded Mfrags/s @1080p (primitive mix)",
+        "metric": f"shaded Mfrags/s @1080p ({names[args.workload]})",
         "value": round(value, 3),
         "unit": "shaded Mfrags/s",
         "n_gpus": 1,
@@ -533,24 +683,20 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded UV sphere + noise texture)" if args.workload == "mesh" else "synthetic (seeded clip-space primitives)",
-        "config": {
-            "workload": "indexed textured UV sphere 256x512 segs, 262,144 tris, 1920x1080, scissor 75%, stencil INCR, depth LESS"
-            if args.workload == "mesh" else f"{args.tris} tris + {args.lines} lines + {args.points} points, 1920x1080, LEQUAL, cull back",
-            "width": args.width, "height": args.height,
-            "tile": int(gpu.srpHipGetTileSize()),
-        },
+        "data": {"mesh": "synthetic (seeded UV sphere + noise texture)", "mix": "synthetic (seeded clip-space primitives)",
+                 "tris": "synthetic (seeded clip-space triangles)",
+                 "terrain": "synthetic (seeded fBm heightfield)"}[args.workload],
+        "config": {"workload": describe(wl), "width": wl["w"], "height": wl["h"], "tile": tile},
         "frame_ms": round(ms_per_step, 4),
-        "triangles_per_s": round(in_tris / (ms_per_step * 1e-3), 1),
+        "triangles_per_s": round(input_triangles(wl) / (ms_per_step * 1e-3), 1),
         "shaded_per_frame": int(shaded_per_frame),
-        "covered_per_frame": int(covered_per_frame),
-        "primitives_per_frame": int(prims_per_frame),
-        "draws_per_frame": int(draws_per_frame),
+        "primitives_per_frame": int(g["primitives_per_frame"]),
+        "draws_per_frame": int(g["draws_per_frame"]),
         "kernel_ms": {
             "raster_tris": round(raster_ms, 4),
-            "draw_total": round(ms[1] / max(launches[1], 1), 4),
-            "fragment_lists": round(ms[2] / max(launches[2], 1), 4) if launches[2] else None,
-            "vertex_and_count_pass": round(ms[3] / max(launches[3], 1), 4),
+            "draw_total": round(g["draw_ms"], 4),
+            "fragment_lists": round(g["frag_ms"], 4) if g["frag_ms"] is not None else None,
+            "vertex_and_count_pass": round(g["front_ms"], 4),
         },
         "roofline": {
             "bound": "hbm",
@@ -560,7 +706,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
-            "algorithmic_bytes": int(raster_bytes),
+            "algorithmic_bytes": int(rbytes),
             "frame_bytes": int(frame_bytes),
             "frame_achieved": round(frame_bytes / (ms_per_step * 1e-3) / 1e9, 2),
         },
@@ -568,18 +714,19 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        # the oracle (a C restatement of the reference) checks the frame; the
-        # compiled reference, where it was built, is the timed baseline
+        # the oracle (a C restatement of the reference) checks the frame and
+        # counts covered fragments and box pixels; the compiled reference,
+        # where it was built, is the timed baseline
         have_ref = os.path.exists(REF_SO)
         port_budget = min(args.cpu_budget, 3.0) if have_ref else args.cpu_budget
-        cpu, cframe, cpu_shaded = cpu_baseline(args, tex, port_budget)
+        cpu, cframe, cpu_shaded = cpu_baseline(args, wl, tex, port_budget)
         bbox_per_frame = cpu["bbox_pixels_per_frame"]
-        same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(gframe, cframe))
-        result["parity_vs_oracle"] = bool(same and cpu_shaded == shaded_per_frame)
+        covered_per_frame = cpu["covered_per_frame"]
+        result["covered_per_frame"] = int(covered_per_frame)
+        result["parity_vs_oracle"] = bool(frames_equal(gframe, cframe) and cpu_shaded == shaded_per_frame)
         if have_ref:
-            cpu, rframe = reference_baseline(args, tex, args.cpu_budget, cpu)
-            result["parity_vs_reference"] = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
-                                                for a, b in zip(gframe, rframe))
+            cpu, rframe = reference_baseline(args, wl, tex, args.cpu_budget, cpu)
+            result["parity_vs_reference"] = frames_equal(gframe, rframe)
         if args.workload == "mesh" and cpu_shaded == shaded_per_frame:
             result["valu"] = valu_figure(bbox_per_frame, covered_per_frame, shaded_per_frame,
                                          raster_ms, ms_per_step)
@@ -588,6 +735,14 @@ def main():
         result["cpu_baseline"] = cpu
         result["speedup_vs_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
 
+    if args.workload == "mesh" and not args.no_legs:
+        cpu_on = not args.no_cpu_baseline
+        print("bench.py: primitive-mix leg (1080p, 500k tris + 100k lines + 100k points)", file=sys.stderr, flush=True)
+        result["mix"] = workload_leg(args, tex, "mix", 10, 2, 1.0 / 25.0, 5.0 if cpu_on else 0.0, 1, 3)
+        print("bench.py: random-triangle leg (512x512, 2,000 tris)", file=sys.stderr, flush=True)
+        result["tris512"] = workload_leg(args, tex, "tris", 50, 3, 1.0, 4.0 if cpu_on else 0.0, 1, 5)
+        print("bench.py: terrain leg (1080p, 1M tris, procedural material)", file=sys.stderr, flush=True)
+        result["terrain"] = workload_leg(args, tex, "terrain", 50, 3, 1.0, 3.0 if cpu_on else 0.0, 2, 8)
     if not args.no_batch and args.workload == "mesh":
         print("bench.py: batched-scenes leg (64 scenes, autotune, graph)", file=sys.stderr, flush=True)
         result["batch"] = batch_leg(args)

@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 9u
+#define SRP_ABI_VERSION 10u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -181,6 +181,10 @@ typedef struct SrpRasterLaunch
 	uint32_t batched;           /* batched three-phase kernel (tile 8, one wave per tile) */
 	uint32_t bigPixel;          /* batched: triangles without exact masks are shaded
 	                               pixel-parallel in runs instead of via fragment lists */
+	uint32_t depthCull;         /* batched runs: skip a triangle for a whole tile when
+	                               every fragment it could have there fails the depth
+	                               test (no stencil, no late depth); the fragments so
+	                               skipped are not counted as covered */
 	/* multi-draw persistent mode (nSeg > 0): nTiles counts the tiles of all
 	 * draws, seg[d] covers draw d's tiles [first, first + count), and the
 	 * nBlocks waves take tiles from `queues` (8 counters, one per range of

@@ -20,6 +20,7 @@
 #include "srp/shaders.h"
 #include "srp/device/abi.h"
 #include "srp/device/replay.h"
+#include "srp/device/depth_cull.h"
 
 #pragma clang fp contract(off)
 
@@ -591,7 +592,7 @@ __device__ __forceinline__ uint64_t takeRound(uint64_t& rest)
 	return round;
 }
 
-template <class FS, bool LATE, bool WORDS>
+template <class FS, bool LATE, bool WORDS, bool CULL>
 __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, const SrpDrawParams* P, uint32_t tile,
                                                   uint32_t gtile, uint32_t& covered, uint32_t& shaded)
 {
@@ -628,6 +629,10 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	}
 	const uint32_t pv = st.provokingLast ? 2u : 0u;
 	const bool precull = !LATE && st.depthTest && monotoneOp(st.depthOp);
+	/* whole-tile depth culling of pixel-parallel runs: a culled fragment
+	 * must have no effect at all, so no stencil (its dfail op) and no
+	 * shader-written depth */
+	const bool cull = CULL && L.depthCull && precull && !st.stencilEnabled;
 
 	/* The tile's list is consumed through a window of 64 slots: slot s
 	 * (lane s keeps its mask, recIdx[s] its record) holds list entry
@@ -690,24 +695,58 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		const uint64_t unkWin = __ballot(lane < avail && !known);
 		if (L.bigPixel && (unkWin & 1ull))
 		{
-			/* A run of up to 8 triangles without exact masks heads the
-			 * window: lane = pixel, the triangles in order, each fragment
-			 * tested and shaded as the reference does, with no fragment
-			 * list (these triangles cover much of the tile). */
+			/* A run of triangles without exact masks heads the window:
+			 * lane = pixel, the triangles in order, each fragment tested
+			 * and shaded as the reference does, with no fragment list
+			 * (these triangles cover much of the tile).  Triangles whose
+			 * every fragment here must fail the depth test are dropped
+			 * first (srpTileDepthCull); the run then takes window entries
+			 * until it holds 8 triangles that remain. */
 			const uint64_t notUnk = ~unkWin;
-			uint32_t run = notUnk ? (uint32_t) __builtin_ctzll(notUnk) : 64u;
-			if (run > 8u)
-				run = 8u;
-			rowBases(L.tris, recIdx, head, (1ull << run) - 1ull, tx, ty, lane, base);
-			__syncthreads();
-			for (uint32_t j = 0; j < run; j++)
+			const uint32_t runMax = notUnk ? (uint32_t) __builtin_ctzll(notUnk) : 64u;
+			uint64_t keep = (runMax >= 64u) ? ~0ull : ((1ull << runMax) - 1ull);
+			if (cull)
 			{
+				/* the tile's stored depth range (pixels inside the
+				 * framebuffer); a run's culls use its value at the run's
+				 * start, which is conservative because a monotone compare
+				 * only writes depths that make later fragments fail more */
+				float zMax = inFb ? dep : -__builtin_inff();
+				float zMin = inFb ? dep : __builtin_inff();
+				for (int off = 32; off > 0; off >>= 1)
+				{
+					zMax = fmaxf(zMax, __shfl_xor(zMax, off));
+					zMin = fminf(zMin, __shfl_xor(zMin, off));
+				}
+				bool drop = false;
+				if (lane < runMax)
+					drop = srpTileDepthCull(&L.tris[recIdx[(head + lane) & 63u]], tx, ty, st.depthOp, zMin, zMax);
+				keep &= ~__ballot(drop);
+			}
+			uint32_t run = runMax;
+			{
+				uint64_t k = keep;
+				for (int c = 0; c < 8 && k; c++)
+					k &= k - 1ull;
+				if (k)
+					run = (uint32_t) __builtin_ctzll(k);   /* the 9th remaining entry starts the next run */
+			}
+			const uint64_t keepRun = keep & ((run >= 64u) ? ~0ull : ((1ull << run) - 1ull));
+			if (keepRun)
+			{
+				rowBases(L.tris, recIdx, head, keepRun, tx, ty, lane, base);
+				__syncthreads();
+			}
+			uint32_t jb = 0;   /* row-base slot: rank of the triangle among those kept */
+			for (uint64_t rest = keepRun; rest; rest &= rest - 1ull, jb++)
+			{
+				const uint32_t j = (uint32_t) __builtin_ctzll(rest);
 				const SrpTri& T = L.tris[recIdx[(head + j) & 63u]];
 				bool cov = px >= (int) T.x0 && px < (int) T.x1 && py >= (int) T.y0 && py < (int) T.y1;
 				float l[3] = {0.0f, 0.0f, 0.0f};
 				if (cov)
 				{
-					baryFromBase(T, base, j, px, py, tx, l);
+					baryFromBase(T, base, jb, px, py, tx, l);
 					cov = insideTri(T, l);
 				}
 				const bool front = (T.flags >> 3) & 1u;
@@ -1021,14 +1060,14 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	}
 }
 
-template <class FS, bool LATE, bool WORDS, int WAVES>
+template <class FS, bool LATE, bool WORDS, int WAVES, bool CULL>
 __global__ __launch_bounds__(64, WAVES) void srp_raster_batched(SrpRasterLaunch L)
 {
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
 	if (tile >= L.nTiles)
 		return;
 	uint32_t covered = 0, shaded = 0;
-	rasterTileBatched<FS, LATE, WORDS>(L, L.params, tile, tile, covered, shaded);
+	rasterTileBatched<FS, LATE, WORDS, CULL>(L, L.params, tile, tile, covered, shaded);
 	flushStats(L.params, covered, shaded);
 }
 
@@ -1038,7 +1077,7 @@ __global__ __launch_bounds__(64, WAVES) void srp_raster_batched(SrpRasterLaunch 
  * split into 8 contiguous ranges, one per XCD (blocks are dispatched to
  * XCDs round-robin), and a wave that drains its own range moves on to the
  * next; otherwise one queue serves all. */
-template <class FS, bool LATE, bool WORDS, int WAVES>
+template <class FS, bool LATE, bool WORDS, int WAVES, bool CULL>
 __global__ __launch_bounds__(64, WAVES) void srp_raster_persistent(SrpRasterLaunch L)
 {
 	const uint32_t nq = L.xcdQueues ? 8u : 1u;
@@ -1067,7 +1106,7 @@ __global__ __launch_bounds__(64, WAVES) void srp_raster_persistent(SrpRasterLaun
 				else b = m;
 			}
 			const SrpSeg sg = L.seg[a];
-			rasterTileBatched<FS, LATE, WORDS>(L, sg.P, t - sg.first, t, covered, shaded);
+			rasterTileBatched<FS, LATE, WORDS, CULL>(L, sg.P, t - sg.first, t, covered, shaded);
 			last = sg.P;
 		}
 	}
@@ -1167,14 +1206,27 @@ void launchRasterT(const SrpRasterLaunch* L, hipStream_t s)
 		hipLaunchKernelGGL((srp_raster_kernel<FS, LATE, WORDS, 8>), dim3(L->nBlocks), dim3(256), 0, s, *L);
 }
 
+/* Depth culling has its own kernels (CULL), so that draws it cannot help
+ * (stencil on, late depth, culling off) run kernels without its registers. */
 template <class FS, bool LATE, bool WORDS>
 void launchBatchedT(const SrpRasterLaunch* L, hipStream_t s)
 {
 	constexpr int W = FsIsWave<FS>::value ? 2 : SRP_RASTER_WAVES;
+	const bool cull = !LATE && L->depthCull;
 	if (L->nSeg)
-		hipLaunchKernelGGL((srp_raster_persistent<FS, LATE, WORDS, W>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+	{
+		if (cull)
+			hipLaunchKernelGGL((srp_raster_persistent<FS, LATE, WORDS, W, !LATE>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+		else
+			hipLaunchKernelGGL((srp_raster_persistent<FS, LATE, WORDS, W, false>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+	}
 	else
-		hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS, W>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+	{
+		if (cull)
+			hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS, W, !LATE>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+		else
+			hipLaunchKernelGGL((srp_raster_batched<FS, LATE, WORDS, W, false>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+	}
 }
 
 template <class FS>

@@ -35,6 +35,11 @@ SRP_EXTERN_C_BEGIN
 void srpFramebufferSetDeferred(const SRPFramebuffer* fb, bool deferred);
 void srpFramebufferDownload(const SRPFramebuffer* fb);
 void srpFramebufferUpload(const SRPFramebuffer* fb);
+/** Write the colour buffer as an RGBA PNG, alpha 255 (what the reference's
+ *  test harness does with its frames: tests/utils/save.c:4-33).  Deferred
+ *  framebuffers are downloaded first.  0 on success, else -1 and a
+ *  message. */
+int srpFramebufferSaveImage(const SRPFramebuffer* fb, const char* path);
 /** Fill the whole depth / stencil buffer (device side in deferred mode). */
 void srpFramebufferFillDepth(const SRPFramebuffer* fb, float value);
 void srpFramebufferFillStencil(const SRPFramebuffer* fb, uint8_t value);
@@ -75,6 +80,15 @@ int srpHipGetTileSize(void);
  *  maskBox: exact 8x8 binning masks for triangle boxes up to this side
  *  (0..24).  A negative argument leaves that setting unchanged. */
 void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox);
+/** Tile-level depth culling (on by default): in the batched kernel, a
+ *  triangle none of whose fragments in an 8x8 tile can pass the depth test
+ *  (a conservative bound on its interpolated depth against the tile's
+ *  stored depth range; LESS/LEQUAL/GREATER/GEQUAL/NEVER, stencil off, the
+ *  shader not writing depth) is skipped for that tile.  The buffers and the
+ *  shaded count are unchanged; the covered count then omits the skipped
+ *  fragments (exact again with culling off). */
+void srpHipSetTileDepthCull(int enable);
+int srpHipGetTileDepthCull(void);
 
 /** Shader registration (see srp/srp_device.h).  `entry` points to a
  *  struct SrpVsEntry / SrpFsEntry of srp/device/abi.h. */
@@ -111,20 +125,30 @@ typedef struct SRPBatchTuning
 	int maskBox;      /**< exact 8x8 binning masks for triangle boxes up to this side (0..24 px) */
 	int xcdQueues;    /**< 1: one tile queue per XCD range (work stealing between them); 0: one queue */
 	int pixelRuns;    /**< 1: triangles without exact masks are shaded pixel-parallel in runs; 0: via fragment lists */
+	int depthCull;    /**< 1: tile-level depth culling (see srpHipSetTileDepthCull; covered counts become a lower bound) */
 } SRPBatchTuning;
 
 typedef struct SRPBatch SRPBatch;
 
 /** Validate and prepare the draws (device memory is sized by one counting
- *  pass over them).  NULL, with a message, on invalid input.  The draws'
- *  uniforms, vertex and index data are read when the batch is made and by
- *  every frame; the commands themselves are copied. */
+ *  pass over them; nothing is drawn).  NULL, with a message, on invalid
+ *  input.  The commands and each draw's render state (its context, or the
+ *  current one) are copied, and so are the uniforms' contents; vertex and
+ *  index data are read by every frame.  Giving a vertex or index buffer
+ *  new data (srp*BufferCopyData) makes the next frame prepare the batch
+ *  again from the copied commands and state (re-reading the uniforms,
+ *  keeping the tuning, capturing the graph again if there was one);
+ *  freeing a buffer or framebuffer the batch uses makes its frames draw
+ *  nothing, with a message.  Errors a frame meets on the device (a clipped
+ *  polygon too large, a capacity exceeded) are reported through the message
+ *  callback by the next batch call. */
 SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n);
 void srpFreeBatch(SRPBatch* batch);
 /** Render one frame of every draw in command order (a replay of the HIP
  *  graph once captured).  Queued on the library stream. */
 void srpBatchRender(SRPBatch* batch);
-/** Capture the frame into a HIP graph; later frames replay it.  0 on success. */
+/** Capture the frame into a HIP graph (nothing is drawn now); later frames
+ *  replay it.  0 on success. */
 int srpBatchCapture(SRPBatch* batch);
 /** Change the tuning (drops a captured graph) / read it. */
 void srpBatchSetTuning(SRPBatch* batch, const SRPBatchTuning* tuning);

@@ -36,6 +36,7 @@
 #define SRP_INCLUDE_MAT
 #include "srp/srp.h"
 #include "png_decode.h"
+#include "png_encode.h"
 
 /* vec/mat helpers: same definitions the library exports. */
 #define SRP_MATH_FN
@@ -249,6 +250,14 @@ SRPFramebuffer* srpNewFramebuffer(size_t w, size_t h)
 	return fb;
 }
 void srpFreeFramebuffer(SRPFramebuffer* fb) { if (fb) { free(fb->color); free(fb->depth); free(fb->stencil); free(fb); } }
+
+/* The test harness's frame writer (reference tests/utils/save.c:4-33). */
+int srpFramebufferSaveImage(const SRPFramebuffer* fb, const char* path)
+{
+	if (!fb || !path || fb->width > (1u << 24) || fb->height > (1u << 24))
+		return -1;
+	return srpPngSaveColorBuffer(path, (int) fb->width, (int) fb->height, fb->color);
+}
 /* core/framebuffer.c:57-62: colour 0, depth -1, stencil untouched. */
 void srpFramebufferClear(const SRPFramebuffer* fb)
 {

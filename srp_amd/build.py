@@ -64,7 +64,7 @@ def build_library(jobs=8):
     os.makedirs(OBJ, exist_ok=True)
     inc = [INCLUDE, os.path.join(CSRC, "lib"), os.path.join(CSRC, "host")]
     hip_srcs = ["lib/runtime.cpp", "lib/assemble.hip", "lib/util.hip", "neural/mlp.hip"]
-    c_srcs = ["host/math.c", "host/png_decode.c", "host/obj_load.c"]
+    c_srcs = ["host/math.c", "host/png_decode.c", "host/png_encode.c", "host/obj_load.c"]
     tasks = []
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for s in hip_srcs:
@@ -107,9 +107,50 @@ def build_scenes(jobs=8):
     """The GPU build of the test scene programs (tests/scenes)."""
     host = os.path.join(CSRC, "host")
     return build_shader_module(
-        [os.path.join(SCENE_DIR, "device_shaders.hip"), os.path.join(SCENE_DIR, "copy_probe.hip")],
+        [os.path.join(SCENE_DIR, f) for f in ("device_shaders.hip", "device_shaders_1.hip", "device_shaders_2.hip",
+                                              "device_shaders_3.hip", "device_shaders_4.hip", "copy_probe.hip")],
         [os.path.join(SCENE_DIR, f) for f in ("scenes.c", "fuzz.c", "bench.c", "messages.c", "terrain.c", "batch.c")] + [os.path.join(host, "obj_load.c")],
         SCENE_LIB, includes=[SCENE_DIR, host], jobs=jobs, defines=["SRP_GPU_BUILD"])
+
+
+INTEGRATION_DIR = os.path.join(REPO, "tests", "integration")
+INTEGRATION_BUILD = os.path.join(INTEGRATION_DIR, "build")
+
+
+def integration_sample_sources(doc=os.path.join(REPO, "INTEGRATION.md")):
+    """The sample program's files as INTEGRATION.md shows them: every fenced
+    block preceded by a `<!-- sample: NAME -->` line, indentation removed."""
+    import re
+    import textwrap
+    text = open(doc).read()
+    files = {}
+    for m in re.finditer(r"<!-- sample: (\S+) -->\s*\n(\s*)```\w*\n(.*?)\n\s*```", text, flags=re.S):
+        files[m.group(1)] = textwrap.dedent(m.group(3)) + "\n"
+    return files
+
+
+def build_integration_sample():
+    """INTEGRATION.md's sample, verbatim: the GPU program (shaders.hip as a
+    shader module, main.c linked to libsrp_amd.so) and the same main.c with
+    tests/integration/host_shaders.c against the CPU oracle."""
+    src = os.path.join(INTEGRATION_BUILD, "src")
+    os.makedirs(src, exist_ok=True)
+    for name, body in integration_sample_sources().items():
+        with open(os.path.join(src, name), "w") as f:
+            f.write(body)
+    inc = ["-I", INCLUDE, "-I", src]
+    shaders = _hip_obj(os.path.join(src, "shaders.hip"), os.path.join(INTEGRATION_BUILD, "shaders.o"), [INCLUDE, src])
+    main_o = os.path.join(INTEGRATION_BUILD, "main.o")
+    _run([CC, "-std=c11", "-O2", "-Wall"] + NUMERIC_FLAGS + inc + ["-c", os.path.join(src, "main.c"), "-o", main_o])
+    rpath = os.path.relpath(PKG, INTEGRATION_BUILD)
+    _run([HIPCC, f"--offload-arch={ARCH}", main_o, shaders, "-L", PKG, "-lsrp_amd", f"-Wl,-rpath,$ORIGIN/{rpath}",
+          "-o", os.path.join(INTEGRATION_BUILD, "sample_gpu")])
+    host = os.path.join(CSRC, "host")
+    _run([CC, "-std=c11", "-O2", "-Wall", "-Wno-unused-function"] + NUMERIC_FLAGS + inc + ["-I", host] +
+         [os.path.join(src, "main.c"), os.path.join(INTEGRATION_DIR, "host_shaders.c"),
+          os.path.join(REPO, "oracle", "srp_oracle.c"), os.path.join(host, "png_decode.c"),
+          os.path.join(host, "png_encode.c"), "-lz", "-lm", "-o", os.path.join(INTEGRATION_BUILD, "sample_oracle")])
+    return os.path.join(INTEGRATION_BUILD, "sample_gpu")
 
 
 def build_oracle():
@@ -124,9 +165,11 @@ def build_all(jobs=8):
     build_library(jobs)
     build_scenes(jobs)
     build_oracle()
+    build_integration_sample()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"all": build_all, "lib": build_library, "scenes": build_scenes, "oracle": build_oracle}[what]()
+    {"all": build_all, "lib": build_library, "scenes": build_scenes, "oracle": build_oracle,
+     "sample": build_integration_sample}[what]()
     print("built", what)

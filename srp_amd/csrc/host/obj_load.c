@@ -53,13 +53,19 @@ bool srpObjLoad(const char* path, SRPObjMesh* mesh)
 				fprintf(stderr, "Unsupported face format: %s\n", line);
 				continue;
 			}
+			/* all three corners are checked before any is pushed, so a bad
+			 * face is skipped whole and later triangles stay aligned */
+			int ok = 1;
+			for (int k = 0; k < 3; k++)
+				if (v[k] < 1 || (size_t) v[k] > pos.n || t[k] < 1 || (size_t) t[k] > uv.n || n[k] < 1 || (size_t) n[k] > nrm.n)
+					ok = 0;
+			if (!ok)
+			{
+				fprintf(stderr, "Face index out of range: %s\n", line);
+				continue;
+			}
 			for (int k = 0; k < 3; k++)
 			{
-				if (v[k] < 1 || (size_t) v[k] > pos.n || t[k] < 1 || (size_t) t[k] > uv.n || n[k] < 1 || (size_t) n[k] > nrm.n)
-				{
-					fprintf(stderr, "Face index out of range: %s\n", line);
-					break;
-				}
 				SRPObjVertex* ov = (SRPObjVertex*) growPush(&verts);
 				ov->position = ((vec3*) pos.p)[v[k] - 1];
 				ov->uv = ((vec2*) uv.p)[t[k] - 1];

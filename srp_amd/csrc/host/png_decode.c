@@ -81,6 +81,9 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 		pos += 12 + (size_t) len;
 	}
 	if (w == 0 || h == 0 || ctype < 0) return fail(err, "missing IHDR", file, idat);
+	/* as stb_image (the reference's loader): sides above 2^24 are refused,
+	 * so no size product below can overflow or truncate */
+	if (w > (1u << 24) || h > (1u << 24)) return fail(err, "image too large", file, idat);
 	if (interlace) return fail(err, "interlaced PNG not supported", file, idat);
 
 	int channels;
@@ -99,6 +102,11 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 	const size_t bitsPerPixel = (size_t) channels * (size_t) depth;
 	const size_t rowBytes = ((size_t) w * bitsPerPixel + 7) / 8;
 	const size_t bpp = (bitsPerPixel + 7) / 8;  /* filter byte distance */
+	/* w, h <= 2^24 and at most 64 bits per pixel: (rowBytes + 1) * h and
+	 * w * h * 3 stay below 2^55; zlib's length type must hold the former,
+	 * and the output's side lengths must fit an int */
+	if ((rowBytes + 1) * (size_t) h > (size_t) (uLongf) -1 || (size_t) w * h * 3 > ((size_t) 1 << 48))
+		return fail(err, "image too large", file, idat);
 	uLongf rawLen = (uLongf) ((rowBytes + 1) * h);
 	uint8_t* raw = (uint8_t*) malloc(rawLen);
 	if (!raw) return fail(err, "out of memory", file, idat);

@@ -86,7 +86,22 @@ __device__ __forceinline__ uint64_t loadIndex(const uint8_t* ib, uint32_t elem, 
 __device__ __forceinline__ uint32_t slotOf(const AsmDraw& W, uint64_t stream)
 {
 	const uint64_t vi = W.indexElem ? loadIndex(W.indices, W.indexElem, stream) : stream;
-	return (uint32_t) (vi - W.P->cacheBase);
+	const uint64_t slot = vi - W.P->cacheBase;
+	return slot < (uint64_t) W.P->cacheSize ? (uint32_t) slot : 0xFFFFFFFFu;
+}
+
+/* A vertex index outside the range the draw's post-VS cache was prepared
+ * for (only possible when a batch's index data changed after it was made):
+ * the primitive is dropped and error bit 2 is raised, never read out of
+ * bounds. */
+__device__ __forceinline__ bool slotsValid(const AsmArgs& A, const SrpDrawParams* P, const uint32_t* slot, int n)
+{
+	bool ok = true;
+	for (int i = 0; i < n; i++)
+		ok = ok && slot[i] < P->cacheSize;
+	if (!ok)
+		atomicOr(A.error, 4u);
+	return ok;
 }
 
 /* The draw a block of a (possibly multi-draw) assembly launch works on:
@@ -109,166 +124,41 @@ __device__ __forceinline__ void loadClip(const SrpDrawParams* P, uint32_t slot, 
 	c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
 }
 
-/* ------------------------------------------------------------ clip pool */
+/* ------------------------------------------------------- clip varyings */
 
-/* Vertices produced while clipping one primitive: the originals first, then
- * each generated vertex as (parent a, parent b, t) with position
- * a*(1-t) + b*t (clipping.c:244-258). */
-struct Pool
+/* Varyings of a vertex generated at parameter t between vertices whose
+ * varyings are va and vb, by the clip rule (interpolateAttributes with no
+ * 1/w, interpolation.c:116-120, weights {1-t, t}, clipping.c:244-258):
+ * every float and double element blended affinely, FLAT ones included;
+ * other types copied from the provoking end (vb for LAST, va for FIRST). */
+__device__ void lerpVaryings(const SrpDrawParams* P, const uint8_t* va, const uint8_t* vb, float t, uint8_t* dst)
 {
-	float c[SRP_POOL][4];
-	uint8_t a[SRP_POOL], b[SRP_POOL];
-	float t[SRP_POOL];
-	uint32_t slot[3];
-	int n;
-};
-
-__device__ __forceinline__ int poolLerp(Pool& pl, int ia, int ib, float t)
-{
-	const int k = pl.n;
-	if (k >= SRP_POOL)
-		return -1;
-	for (int i = 0; i < 4; i++)
-		pl.c[k][i] = pl.c[ia][i] * (1 - t) + pl.c[ib][i] * t;
-	pl.a[k] = (uint8_t) ia;
-	pl.b[k] = (uint8_t) ib;
-	pl.t[k] = t;
-	pl.n = k + 1;
-	return k;
-}
-
-/* One Sutherland-Hodgman pass (clipping.c:199-242).  An edge that crosses
- * the plane with |da-db| <= 1e-9 contributes nothing, not even its inside
- * endpoint (clipping.c:226-227).  Returns the new count, -1 on overflow. */
-__device__ int clipPass(Pool& pl, const uint8_t* in, int n, int plane, uint8_t* out)
-{
-	int m = 0;
-	for (int i = 0; i < n; i++)
-	{
-		const int cur = in[i], nxt = in[(i + 1) % n];
-		const float da = planeDist(pl.c[cur], plane);
-		const float db = planeDist(pl.c[nxt], plane);
-		const bool ci = da >= 0.0f, ni = db >= 0.0f;
-		if (ci && ni)
-			out[m++] = (uint8_t) nxt;
-		else if (ci || ni)
-		{
-			if (roughlyZero(da - db))
-				continue;
-			const float t = da / (da - db);
-			const int k = poolLerp(pl, cur, nxt, t);
-			if (k < 0)
-				return -1;
-			out[m++] = (uint8_t) k;
-			if (!ci && ni)
-				out[m++] = (uint8_t) nxt;
-		}
-		if (m + 2 > SRP_MAX_POLY)
-			return -1;
-	}
-	return m;
-}
-
-/* clipping.c:68-119: the clipped polygon as pool indices; -1 on overflow. */
-__device__ int clipTriangle(Pool& pl, uint8_t* poly)
-{
-	const uint32_t c0 = outcode(pl.c[0]), c1 = outcode(pl.c[1]), c2 = outcode(pl.c[2]);
-	poly[0] = 0; poly[1] = 1; poly[2] = 2;
-	if ((c0 | c1 | c2) == 0)
-		return 3;
-	if ((c0 & c1 & c2) != 0)
-		return 0;
-	uint8_t tmp[SRP_MAX_POLY];
-	uint8_t* src = poly;
-	uint8_t* dst = tmp;
-	int n = 3;
-	for (int p = 0; p < 6; p++)
-	{
-		n = clipPass(pl, src, n, p, dst);
-		if (n <= 0)
-			return n;
-		uint8_t* s = src; src = dst; dst = s;
-	}
-	if (src != poly)
-		for (int i = 0; i < n; i++) poly[i] = src[i];
-	return n;
-}
-
-/* ----------------------------------------------------- generated varyings */
-
-/* Value of one float element for pool vertex k under the clip rule
- * (interpolation with invW == NULL: every float varying is blended
- * affinely, interpolation.c:116-120, weights {1-t, t}).  Parents always
- * precede their children in the pool, so one forward sweep suffices. */
-__device__ float genFloat(const SrpDrawParams* P, const Pool& pl, int k, uint32_t off)
-{
-	float val[SRP_POOL];
-	for (int q = 0; q <= k; q++)
-	{
-		if (q < 3)
-			val[q] = srp::dev::loadF(srp::dev::varyingPtr(P, pl.slot[q]) + off);
-		else
-		{
-			const float t = pl.t[q];
-			const float w0 = 1 - t;
-			float v = 0.0f;
-			v += val[pl.a[q]] * w0;
-			v += val[pl.b[q]] * t;
-			val[q] = v;
-		}
-	}
-	return val[k];
-}
-
-__device__ double genDouble(const SrpDrawParams* P, const Pool& pl, int k, uint32_t off)
-{
-	double val[SRP_POOL];
-	for (int q = 0; q <= k; q++)
-	{
-		if (q < 3)
-			val[q] = srp::dev::loadD(srp::dev::varyingPtr(P, pl.slot[q]) + off);
-		else
-		{
-			const float t = pl.t[q];
-			const float w0 = 1 - t;
-			double v = 0.0;
-			v += val[pl.a[q]] * (double) w0;
-			v += val[pl.b[q]] * (double) t;
-			val[q] = v;
-		}
-	}
-	return val[k];
-}
-
-/* Integers come from the pair's provoking vertex: b for LAST, a for FIRST. */
-__device__ __forceinline__ int genIntSource(const SrpDrawParams* P, const Pool& pl, int k)
-{
-	while (k >= 3)
-		k = P->st.provokingLast ? pl.b[k] : pl.a[k];
-	return k;
-}
-
-__device__ void materialize(const SrpDrawParams* P, const Pool& pl, int k, uint8_t* dst)
-{
+	const float w0 = 1 - t;
 	for (uint32_t a = 0; a < P->nVaryings; a++)
 	{
 		const SrpVaryingDesc d = P->var[a];
 		if (d.type == SRP_FLOAT)
 			for (uint32_t e = 0; e < d.nItems; e++)
 			{
-				const float v = genFloat(P, pl, k, d.offset + 4 * e);
-				__builtin_memcpy(dst + d.offset + 4 * e, &v, 4);
+				const uint32_t off = d.offset + 4 * e;
+				float v = 0.0f;
+				v += srp::dev::loadF(va + off) * w0;
+				v += srp::dev::loadF(vb + off) * t;
+				__builtin_memcpy(dst + off, &v, 4);
 			}
 		else if (d.type == SRP_DOUBLE)
 			for (uint32_t e = 0; e < d.nItems; e++)
 			{
-				const double v = genDouble(P, pl, k, d.offset + 8 * e);
-				__builtin_memcpy(dst + d.offset + 8 * e, &v, 8);
+				const uint32_t off = d.offset + 8 * e;
+				double v = 0.0;
+				v += srp::dev::loadD(va + off) * (double) w0;
+				v += srp::dev::loadD(vb + off) * (double) t;
+				__builtin_memcpy(dst + off, &v, 8);
 			}
 		else
 		{
 			const uint32_t n = srp::dev::elemSize(d.type) * d.nItems;
-			const uint8_t* src = srp::dev::varyingPtr(P, pl.slot[genIntSource(P, pl, k)]) + d.offset;
+			const uint8_t* src = (P->st.provokingLast ? vb : va) + d.offset;
 			for (uint32_t b = 0; b < n; b++) dst[d.offset + b] = src[b];
 		}
 	}
@@ -429,13 +319,49 @@ __device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const Sr
 		}
 		return n;
 	}
-	for (uint32_t ty = ty0; ty <= ty1; ty++)
-		for (uint32_t tx = tx0; tx <= tx1; tx++)
-		{
-			emit(tileBase + ty * P->tilesX + tx, ~0ull, false);
-			n++;
-		}
-	return n;
+	/* every tile of the box, without a mask: written by k_expand_spans */
+	return (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
+}
+
+/* True when binTriangle gives T one unmasked entry per tile of its box
+ * (a span) rather than exact masks. */
+__device__ __forceinline__ bool isSpan(const SrpDrawParams* P, const SrpTri& T)
+{
+	if (T.x1 <= T.x0 || T.y1 <= T.y0)
+		return false;
+	const int bw = (int) T.x1 - (int) T.x0, bh = (int) T.y1 - (int) T.y0;
+	return !(P->tileShift == 3 && bw <= (int) P->maskBox && bh <= (int) P->maskBox);
+}
+
+/* Reserve a span-list entry (one atomic per group of lanes calling
+ * together) and fill it: the pairs of T's box tiles, items item.. in
+ * row-major tile order, record `rec` (0xFFFFFFFF: none). */
+__device__ void appendSpan(const AsmArgs& A, const SrpDrawParams* P, const SrpTri& T, uint32_t tileBase, uint32_t item,
+                           uint32_t rec)
+{
+	const uint64_t act = __ballot(1);
+	const uint32_t lane = __lane_id();
+	const uint32_t leader = (uint32_t) __builtin_ctzll(act);
+	uint32_t at = 0;
+	if (lane == leader)
+		at = atomicAdd(A.spanCount, (uint32_t) __popcll(act));
+	at = (uint32_t) __shfl((int) at, (int) leader) + (uint32_t) __popcll(act & ((1ull << lane) - 1ull));
+	if (at >= A.spanCap)
+	{
+		atomicOr(A.error, 2u);
+		return;
+	}
+	const uint32_t sh = P->tileShift;
+	const uint32_t tx0 = T.x0 >> sh, tx1 = (T.x1 - 1u) >> sh;
+	const uint32_t ty0 = T.y0 >> sh, ty1 = (T.y1 - 1u) >> sh;
+	SrpSpan sp;
+	sp.item = item;
+	sp.rec = rec;
+	sp.keyBase = tileBase + ty0 * P->tilesX + tx0;
+	sp.stride = P->tilesX;
+	sp.nx = tx1 - tx0 + 1u;
+	sp.ny = ty1 - ty0 + 1u;
+	A.spans[at] = sp;
 }
 
 /* line.c:79-86 */
@@ -573,141 +499,105 @@ __device__ __forceinline__ void triStream(uint32_t prim, uint64_t base, uint64_t
 	else { s[0] = base; s[1] = base + k + 1; s[2] = base + k + 2; }
 }
 
-/* Assembled primitives of one clipped polygon v[0..n-1] (fan from v[0],
- * clipping.c:107-116), per polygon mode (primitive_assembly.c:80-128). */
-template <bool EMIT, int NMAX>
-__device__ __forceinline__ void emitFan(const AsmArgs& A, const AsmDraw& W, const Vtx* v, int n, uint32_t recBase,
-                                        uint32_t idBase, uint32_t itemBase, uint32_t& nRec, uint32_t& nItems,
-                                        bool drop = false)
+/* The primitives of one fan triangle (a, b, c) per polygon mode
+ * (primitive_assembly.c:80-128): records from recBase + nRec, items from
+ * itemBase + nItems, both counters advanced.  COUNT_SPANS (counting pass):
+ * spans are also counted into *A.spanCount. */
+template <bool EMIT, bool COUNT_SPANS>
+__device__ __forceinline__ void fanTriangle(const AsmArgs& A, const AsmDraw& W, const Vtx& a, const Vtx& b, const Vtx& c,
+                                            uint32_t recBase, uint32_t idBase, uint32_t itemBase, uint32_t& nRec,
+                                            uint32_t& nItems, bool drop)
 {
 	const SrpDrawParams* P = W.P;
 	const uint32_t mode = P->st.polygonMode;
-#pragma unroll
-	for (int f = 1; f + 1 < NMAX; f++)
+	if (mode == SRP_POLYGON_MODE_FILL)
 	{
-		if (f + 1 >= n)
-			break;
-		const Vtx& a = v[0];
-		const Vtx& b = v[f];
-		const Vtx& c = v[f + 1];
-		if (mode == SRP_POLYGON_MODE_FILL)
+		SrpTri T;
+		if (!setupTriangle(P, a, b, c, recBase + nRec - idBase, T))
+			return;
+		const uint32_t r = recBase + nRec;
+		uint32_t np;
+		if (EMIT)
 		{
-			SrpTri T;
-			if (!setupTriangle(P, a, b, c, recBase + nRec - idBase, T))
-				continue;
-			const uint32_t r = recBase + nRec;
-			uint32_t np;
-			if (EMIT)
-			{
-				/* A triangle beyond the record capacity (or with a clip vertex
-				 * beyond its capacity) keeps its counted pairs, written as
-				 * empty pairs without a record. */
-				const bool fits = r < A.recCap && !drop;
-				if (fits)
-					A.tris[r] = T;
-				else
-					atomicOr(A.error, 2u);
-				uint32_t o = itemBase + nItems;
-				np = binTriangle<true>(P, T, W.tileBase, [&](uint32_t key, uint64_t m, bool known) {
-					if (o < A.itemCap)
-					{
-						A.pairKeys[o] = key;
-						A.pairVals[o] = o;
-						A.pairRec[o] = fits ? (r | (known ? 0x80000000u : 0u)) : 0xFFFFFFFFu;
-						A.pairMask[o] = fits ? m : 0ull;
-					}
-					o++;
-				});
-				if (o > A.itemCap)
-					atomicOr(A.error, 2u);
-			}
+			/* A triangle beyond the record capacity (or with a clip vertex
+			 * beyond its capacity) keeps its counted pairs, written as
+			 * empty pairs without a record. */
+			const bool fits = r < A.recCap && !drop;
+			if (fits)
+				A.tris[r] = T;
 			else
-				np = binTriangle<false>(P, T, W.tileBase, [](uint32_t, uint64_t, bool) {});
-			nRec++;
-			nItems += np;
-		}
-		else if (mode == SRP_POLYGON_MODE_LINE)
-		{
-			/* edges a-b, b-c, c-a; called by name, not through an array of
-			 * vertex pointers, which would put the vertices of every
-			 * triangle of the launch in scratch memory */
-			auto edge = [&](const Vtx& p, const Vtx& q) {
-				SrpLine L;
-				setupLine(P, p, q, recBase + nRec - idBase, L);
-				const uint32_t nf = lineFragCount(L);
-				if (EMIT)
+				atomicOr(A.error, 2u);
+			uint32_t o = itemBase + nItems;
+			np = binTriangle<true>(P, T, W.tileBase, [&](uint32_t key, uint64_t m, bool known) {
+				if (o < A.itemCap)
 				{
-					A.lines[recBase + nRec] = L;
-					emitLineFrags(A, P, L, recBase + nRec, itemBase + nItems);
+					A.pairKeys[o] = key;
+					A.pairVals[o] = o;
+					A.pairRec[o] = fits ? (r | (known ? 0x80000000u : 0u)) : 0xFFFFFFFFu;
+					A.pairMask[o] = fits ? m : 0ull;
 				}
-				nRec++;
-				nItems += nf;
-			};
-			edge(a, b);
-			edge(b, c);
-			edge(c, a);
+				o++;
+			});
+			if (isSpan(P, T))
+			{
+				appendSpan(A, P, T, W.tileBase, o, fits ? r : 0xFFFFFFFFu);
+				o += np;
+			}
+			if (o > A.itemCap)
+				atomicOr(A.error, 2u);
 		}
 		else
 		{
-			auto corner = [&](const Vtx& p) {
-				SrpPoint pt;
-				makePoint(P, p, recBase + nRec - idBase, pt);
-				if (EMIT) A.points[recBase + nRec] = pt;
-				nItems += pointFrags<EMIT>(A, P, pt, recBase + nRec, itemBase + nItems);
-				nRec++;
-			};
-			corner(a);
-			corner(b);
-			corner(c);
+			np = binTriangle<false>(P, T, W.tileBase, [](uint32_t, uint64_t, bool) {});
+			if (COUNT_SPANS && isSpan(P, T))
+			{
+				/* spans of the draw (sizes the span list) */
+				const uint64_t act = __ballot(1);
+				if (__lane_id() == (uint32_t) __builtin_ctzll(act))
+					atomicAdd(A.spanCount, (uint32_t) __popcll(act));
+			}
 		}
+		nRec++;
+		nItems += np;
 	}
-}
-
-/* A triangle that needs clipping: Sutherland-Hodgman over the pool, then
- * the varyings of the generated vertices that survive. */
-template <bool EMIT>
-__device__ __forceinline__ void assembleClipped(const AsmArgs& A, const AsmDraw& W, const uint32_t* slot, uint32_t genBase,
-                                             uint32_t recBase, uint32_t idBase, uint32_t itemBase,
-                                             uint32_t& nRec, uint32_t& nGen, uint32_t& nItems)
-{
-	const SrpDrawParams* P = W.P;
-	Pool pl;
-	for (int i = 0; i < 3; i++)
+	else if (mode == SRP_POLYGON_MODE_LINE)
 	{
-		pl.slot[i] = slot[i];
-		loadClip(P, slot[i], pl.c[i]);
-	}
-	pl.n = 3;
-	uint8_t poly[SRP_MAX_POLY];
-	int n = clipTriangle(pl, poly);
-	if (n < 0)
-	{
-		atomicOr(A.error, 1u);
-		n = 0;
-	}
-	Vtx v[SRP_MAX_POLY];
-	bool drop = false;
-	for (int i = 0; i < n; i++)
-	{
-		const int q = poly[i];
-		for (int c = 0; c < 4; c++) v[i].c[c] = pl.c[q][c];
-		v[i].invW = divide(v[i].c);
-		if (q < 3)
-			v[i].vref = pl.slot[q];
-		else
-		{
-			v[i].vref = P->cacheSize + genBase + nGen;
+		/* edges a-b, b-c, c-a; called by name, not through an array of
+		 * vertex pointers, which would put the vertices of every
+		 * triangle of the launch in scratch memory */
+		auto edge = [&](const Vtx& p, const Vtx& q) {
+			SrpLine L;
+			setupLine(P, p, q, recBase + nRec - idBase, L);
+			const uint32_t nf = lineFragCount(L);
 			if (EMIT)
 			{
-				if (genBase + nGen < A.genCap)
-					materialize(P, pl, q, A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
-				else
-					drop = true;
+				A.lines[recBase + nRec] = L;
+				emitLineFrags(A, P, L, recBase + nRec, itemBase + nItems);
 			}
-			nGen++;
-		}
+			nRec++;
+			nItems += nf;
+		};
+		edge(a, b);
+		edge(b, c);
+		edge(c, a);
 	}
-	emitFan<EMIT, SRP_MAX_POLY>(A, W, v, n, recBase, idBase, itemBase, nRec, nItems, drop);
+	else
+	{
+		auto corner = [&](const Vtx& p) {
+			SrpPoint pt;
+			makePoint(P, p, recBase + nRec - idBase, pt);
+			if (EMIT) A.points[recBase + nRec] = pt;
+			/* up to 42 points of up to W*H fragments each: the count
+			 * saturates rather than wraps, and the host's check of the
+			 * exact total then refuses the draw before any emit */
+			const uint32_t nf = pointFrags<EMIT>(A, P, pt, recBase + nRec, itemBase + nItems);
+			nItems = (nItems + nf < nItems) ? 0xFFFFFFFFu : nItems + nf;
+			nRec++;
+		};
+		corner(a);
+		corner(b);
+		corner(c);
+	}
 }
 
 /* Triangle draws (all polygon modes): primitive_assembly.c:37-135.  The
@@ -729,8 +619,19 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 	Vtx v[3];
 	uint32_t codeOr = 0, codeAnd = 0x3Fu;
 	for (int i = 0; i < 3; i++)
-	{
 		slot[i] = slotOf(W, s[i]);
+	if (!slotsValid(A, P, slot, 3))
+	{
+		if (!EMIT)
+		{
+			A.recOff[g] = 0;
+			A.genOff[g] = 0;
+			A.itemOff[g] = 0;
+		}
+		return;
+	}
+	for (int i = 0; i < 3; i++)
+	{
 		loadClip(P, slot[i], v[i].c);
 		const uint32_t c = outcode(v[i].c);
 		codeOr |= c;
@@ -769,7 +670,7 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 	if (codeOr == 0)
 	{
 		for (int i = 0; i < 3; i++) v[i].invW = divide(v[i].c);
-		emitFan<EMIT, 3>(A, W, v, 3, recBase, idBase, itemBase, nRec, nItems);
+		fanTriangle<EMIT, !EMIT>(A, W, v[0], v[1], v[2], recBase, idBase, itemBase, nRec, nItems, false);
 	}
 	if (!EMIT)
 	{
@@ -793,40 +694,309 @@ __device__ __forceinline__ uint32_t drawOfInput(const AsmDraw* D, uint32_t n, ui
 	return lo;
 }
 
+/* ------------------------------------------------------------- clipping */
+
+/* Lanes that clip one triangle together.  A triangle clipped by the six
+ * planes has at most 9 vertices (a polygon gains at most one per plane);
+ * the limits below only guard numerically degenerate input. */
+constexpr uint32_t CLIP_GROUP = 16;
+
+/* One triangle's Sutherland-Hodgman clip (clipping.c:68-119, :199-258) by
+ * a group of CLIP_GROUP lanes, lane j holding polygon vertex j, so nothing
+ * lives in per-lane scratch memory.  Each of the six passes runs as the
+ * reference's loop does (an all-inside pass rotates the polygon by one):
+ * lane i tests edge (i, i+1) and yields 0, 1 or 2 output vertices; a scan
+ * of those counts places them, and output lane j gathers its source edge's
+ * two vertices and either copies the edge's end or computes the crossing
+ * a*(1-t) + b*t with the reference's operation order.  The recipe of each
+ * pass (source edge, crossing or copy, t) is kept so that the varyings of
+ * generated vertices can be replayed afterwards, element by element, with
+ * the clip rule of interpolation.c:116-120 (floats and doubles blended by
+ * {1-t, t} from their two parents' values, other types from the provoking
+ * parent) - the same values a vertex-by-vertex clip produces.
+ *
+ * A pass that yields SRP_MAX_POLY - 1 or more vertices, or a clip that
+ * generates more than SRP_POOL - 3 vertices, overflows: the triangle is
+ * dropped and error bit 0 raised.
+ *
+ * Then lane f (1 <= f <= n-2) assembles fan triangle (0, f, f+1)
+ * (clipping.c:107-116); its records and items follow those of the lanes
+ * before it (a scan over the group). */
+template <bool EMIT>
+__device__ __forceinline__ void clipGroup(const AsmArgs& A, uint32_t g, bool has, uint32_t gl)
+{
+	const uint32_t d = (has && A.nDraws > 1) ? drawOfInput(A.draws, A.nDraws, g) : 0u;
+	const AsmDraw& W = A.draws[d];
+	const SrpDrawParams* P = W.P;
+	const uint32_t gshift = __lane_id() & ~(CLIP_GROUP - 1u);
+	const uint64_t below = (1ull << gl) - 1ull;
+
+	/* lanes 0..2: the triangle's vertices (slots validated by the counting
+	 * pass of k_assemble_tris before it listed the triangle) */
+	uint32_t mySlot = 0;
+	float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+	if (has && gl < 3u)
+	{
+		uint64_t st[3];
+		triStream(W.prim, W.start, g - W.primBase, st);
+		mySlot = slotOf(W, gl == 0u ? st[0] : (gl == 1u ? st[1] : st[2]));
+		loadClip(P, mySlot, c);
+	}
+	int np = has ? 3 : 0;
+	uint32_t orig = gl;   /* original vertex (0..2) this lane's vertex is, 3: generated */
+	uint32_t genTotal = 0;
+	bool overflow = false;
+	uint32_t rSrc[6], rNxt[6];
+	bool rLerp[6];
+	float rT[6];
+#pragma unroll
+	for (int p = 0; p < 6; p++)
+	{
+		const bool live = (int) gl < np;
+		const uint32_t nxtL = ((int) gl + 1 < np) ? gl + 1u : 0u;
+		const float da = planeDist(c, p);
+		/* Group-uniform shortcuts: with every vertex inside the plane each
+		 * edge yields its end, which rotates the polygon by one; with every
+		 * vertex outside nothing is left.  (The lanes of one group branch
+		 * together, so their shuffles stay within active lanes.) */
+		const uint64_t liveMask = (np >= (int) CLIP_GROUP) ? 0xFFFFull : ((1ull << np) - 1ull);
+		const uint64_t inMask = (__ballot(live && da >= 0.0f) >> gshift) & 0xFFFFull;
+		if (inMask == liveMask)
+		{
+#pragma unroll
+			for (int q = 0; q < 4; q++) c[q] = __shfl(c[q], (int) nxtL, CLIP_GROUP);
+			orig = __shfl(orig, (int) nxtL, CLIP_GROUP);
+			rSrc[p] = gl;
+			rNxt[p] = nxtL;
+			rLerp[p] = false;
+			rT[p] = 0.0f;
+			continue;
+		}
+		if (inMask == 0ull)
+		{
+			np = 0;
+			rSrc[p] = gl;
+			rNxt[p] = gl;
+			rLerp[p] = false;
+			rT[p] = 0.0f;
+			continue;
+		}
+		float nc[4];
+#pragma unroll
+		for (int q = 0; q < 4; q++) nc[q] = __shfl(c[q], (int) nxtL, CLIP_GROUP);
+		const float db = planeDist(nc, p);
+		const bool ci = da >= 0.0f, ni = db >= 0.0f;
+		const bool cross = live && (ci != ni);
+		const bool gen = cross && !roughlyZero(da - db);
+		const float t = gen ? da / (da - db) : 0.0f;
+		const uint32_t cnt = !live ? 0u : ((ci && ni) ? 1u : (!gen ? 0u : ((!ci && ni) ? 2u : 1u)));
+		uint32_t incl = cnt;
+#pragma unroll
+		for (uint32_t off = 1; off < CLIP_GROUP; off <<= 1)
+		{
+			const uint32_t v = __shfl_up(incl, off, CLIP_GROUP);
+			if (gl >= off) incl += v;
+		}
+		const uint32_t m = __shfl(incl, (int) CLIP_GROUP - 1, CLIP_GROUP);
+		genTotal += (uint32_t) __popcll((__ballot(gen) >> gshift) & 0xFFFFull);
+		/* output j comes from the edge i with incl[i-1] <= j < incl[i] */
+		uint32_t src = 0;
+#pragma unroll
+		for (uint32_t step = CLIP_GROUP / 2; step > 0; step >>= 1)
+		{
+			const uint32_t v = __shfl(incl, (int) (src + step - 1u), CLIP_GROUP);
+			if (v <= gl) src += step;
+		}
+		const uint32_t sIncl = __shfl(incl, (int) src, CLIP_GROUP);
+		const uint32_t sCnt = __shfl(cnt, (int) src, CLIP_GROUP);
+		const bool sGen = __shfl((int) gen, (int) src, CLIP_GROUP) != 0;
+		const float sT = __shfl(t, (int) src, CLIP_GROUP);
+		const uint32_t sNxt = __shfl(nxtL, (int) src, CLIP_GROUP);
+		float a[4], b[4];
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+		{
+			a[q] = __shfl(c[q], (int) src, CLIP_GROUP);
+			b[q] = __shfl(c[q], (int) sNxt, CLIP_GROUP);
+		}
+		/* an edge's crossing is its first output; its second (entering
+		 * edges) and an all-inside edge's only output are its end */
+		const bool lerp = sGen && gl + sCnt == sIncl;
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+			c[q] = lerp ? a[q] * (1 - sT) + b[q] * sT : b[q];
+		const uint32_t oNxt = __shfl(orig, (int) sNxt, CLIP_GROUP);
+		orig = lerp ? 3u : oNxt;
+		rSrc[p] = src;
+		rNxt[p] = sNxt;
+		rLerp[p] = lerp;
+		rT[p] = sT;
+		np = (int) m;
+		if (m + 2u > (uint32_t) SRP_MAX_POLY || genTotal > (uint32_t) SRP_POOL - 3u)
+		{
+			overflow = true;
+			np = 0;
+		}
+	}
+	if (overflow && gl == 0u)
+		atomicOr(A.error, 1u);
+
+	/* the polygon's vertices after the divide; generated ones numbered in
+	 * polygon order */
+	const bool live = (int) gl < np;
+	Vtx v;
+#pragma unroll
+	for (int q = 0; q < 4; q++) v.c[q] = c[q];
+	v.invW = live ? divide(v.c) : 0.0f;
+	const uint64_t genLanes = (__ballot(live && orig == 3u) >> gshift) & 0xFFFFull;
+	const uint32_t nGen = (uint32_t) __popcll(genLanes);
+	const uint32_t rank = (uint32_t) __popcll(genLanes & below);
+	const uint32_t origSlot = __shfl(mySlot, (int) (orig < 3u ? orig : 0u), CLIP_GROUP);
+	uint32_t recBase = 0, genBase = 0, itemBase = 0, idBase = 0;
+	if (EMIT && has)
+	{
+		recBase = A.recOff[g];
+		genBase = A.genOff[g];
+		itemBase = A.itemOff[g];
+		idBase = A.recOff[W.primBase];
+	}
+	v.vref = (orig < 3u) ? origSlot : P->cacheSize + genBase + rank;
+
+	bool drop = false;
+	if (EMIT)
+	{
+		const bool mine = live && orig == 3u;
+		const bool over = mine && genBase + rank >= A.genCap;
+		drop = ((__ballot(over) >> gshift) & 0xFFFFull) != 0ull;
+		uint8_t* dst = (mine && !over) ? A.genVaryings + (size_t) (genBase + rank) * P->varyingsSize : nullptr;
+		const uint8_t* own = (has && gl < 3u) ? srp::dev::varyingPtr(P, mySlot) : nullptr;
+		/* integer elements: the original each vertex takes them from */
+		uint32_t root = gl < 3u ? gl : 0u;
+#pragma unroll
+		for (int p = 0; p < 6; p++)
+		{
+			const uint32_t cu = __shfl(root, (int) rSrc[p], CLIP_GROUP), nx = __shfl(root, (int) rNxt[p], CLIP_GROUP);
+			root = rLerp[p] ? (P->st.provokingLast ? nx : cu) : nx;
+		}
+		const uint32_t rootSlot = __shfl(mySlot, (int) root, CLIP_GROUP);
+		for (uint32_t at = 0; at < P->nVaryings; at++)
+		{
+			const SrpVaryingDesc dsc = P->var[at];
+			if (dsc.type == SRP_FLOAT)
+				for (uint32_t e = 0; e < dsc.nItems; e++)
+				{
+					const uint32_t off = dsc.offset + 4 * e;
+					float val = own ? srp::dev::loadF(own + off) : 0.0f;
+#pragma unroll
+					for (int p = 0; p < 6; p++)
+					{
+						const float cu = __shfl(val, (int) rSrc[p], CLIP_GROUP);
+						const float nx = __shfl(val, (int) rNxt[p], CLIP_GROUP);
+						const float w0 = 1 - rT[p];
+						float lv = 0.0f;
+						lv += cu * w0;
+						lv += nx * rT[p];
+						val = rLerp[p] ? lv : nx;
+					}
+					if (dst)
+						__builtin_memcpy(dst + off, &val, 4);
+				}
+			else if (dsc.type == SRP_DOUBLE)
+				for (uint32_t e = 0; e < dsc.nItems; e++)
+				{
+					const uint32_t off = dsc.offset + 8 * e;
+					double val = own ? srp::dev::loadD(own + off) : 0.0;
+#pragma unroll
+					for (int p = 0; p < 6; p++)
+					{
+						const double cu = __shfl(val, (int) rSrc[p], CLIP_GROUP);
+						const double nx = __shfl(val, (int) rNxt[p], CLIP_GROUP);
+						const float w0 = 1 - rT[p];
+						double lv = 0.0;
+						lv += cu * (double) w0;
+						lv += nx * (double) rT[p];
+						val = rLerp[p] ? lv : nx;
+					}
+					if (dst)
+						__builtin_memcpy(dst + off, &val, 8);
+				}
+			else if (dst)
+			{
+				const uint32_t n = srp::dev::elemSize(dsc.type) * dsc.nItems;
+				const uint8_t* from = srp::dev::varyingPtr(P, rootSlot) + dsc.offset;
+				for (uint32_t b = 0; b < n; b++) dst[dsc.offset + b] = from[b];
+			}
+		}
+	}
+
+	/* fan triangle (0, gl, gl + 1) */
+	Vtx a, cc;
+	const uint32_t nextL = gl + 1u < CLIP_GROUP ? gl + 1u : gl;
+#pragma unroll
+	for (int q = 0; q < 4; q++)
+	{
+		a.c[q] = __shfl(v.c[q], 0, CLIP_GROUP);
+		cc.c[q] = __shfl(v.c[q], (int) nextL, CLIP_GROUP);
+	}
+	a.invW = __shfl(v.invW, 0, CLIP_GROUP);
+	cc.invW = __shfl(v.invW, (int) nextL, CLIP_GROUP);
+	a.vref = __shfl(v.vref, 0, CLIP_GROUP);
+	cc.vref = __shfl(v.vref, (int) nextL, CLIP_GROUP);
+	const bool fan = gl >= 1u && (int) gl + 1 < np;
+	uint32_t myRec = 0, myItems = 0;
+	if (fan)
+		fanTriangle<false, !EMIT>(A, W, a, v, cc, 0u, 0u, 0u, myRec, myItems, false);
+	/* exclusive scans over the group (items saturate: see fanTriangle) */
+	uint32_t exRec = myRec;
+	unsigned long long exItems = myItems;
+#pragma unroll
+	for (uint32_t off = 1; off < CLIP_GROUP; off <<= 1)
+	{
+		const uint32_t r = __shfl_up(exRec, off, CLIP_GROUP);
+		const unsigned long long it = __shfl_up(exItems, off, CLIP_GROUP);
+		if (gl >= off)
+		{
+			exRec += r;
+			exItems += it;
+		}
+	}
+	const uint32_t totRec = __shfl(exRec, (int) CLIP_GROUP - 1, CLIP_GROUP);
+	const unsigned long long totItems = __shfl(exItems, (int) CLIP_GROUP - 1, CLIP_GROUP);
+	exRec -= myRec;
+	exItems -= myItems;
+	if (!EMIT)
+	{
+		if (has && gl == 0u)
+		{
+			A.recOff[g] = totRec;
+			A.genOff[g] = nGen;
+			A.itemOff[g] = totItems > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t) totItems;
+		}
+	}
+	else if (fan)
+	{
+		uint32_t r = 0, it = 0;
+		fanTriangle<true, false>(A, W, a, v, cc, recBase + exRec, idBase, itemBase + (uint32_t) exItems, r, it, drop);
+	}
+}
+
 /* The triangles k_assemble_tris listed for clipping, in either pass (the
- * list stays valid from the counting pass to the emitting one).  Each
- * writes only at its own scanned offsets, so list order does not matter. */
+ * list stays valid from the counting pass to the emitting one): one group
+ * of CLIP_GROUP lanes per triangle, the grid striding over the list (every
+ * lane of a block runs the same number of rounds, so the group shuffles
+ * stay convergent).  Each triangle writes only at its own scanned offsets,
+ * so list order does not matter. */
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_clipped(AsmArgs A)
 {
 	const uint32_t n = *A.clipCount;
-	for (uint32_t i = blockIdx.x * 128u + threadIdx.x; i < n; i += gridDim.x * 128u)
+	constexpr uint32_t perBlock = 128u / CLIP_GROUP;
+	const uint32_t gl = threadIdx.x & (CLIP_GROUP - 1u);
+	for (uint32_t base = blockIdx.x * perBlock; base < n; base += gridDim.x * perBlock)
 	{
-		const uint32_t g = A.clipList[i];
-		const uint32_t d = A.nDraws > 1 ? drawOfInput(A.draws, A.nDraws, g) : 0u;
-		const AsmDraw& W = A.draws[d];
-		const uint32_t k = g - W.primBase;
-		uint64_t s[3];
-		triStream(W.prim, W.start, k, s);
-		uint32_t slot[3];
-		for (int j = 0; j < 3; j++)
-			slot[j] = slotOf(W, s[j]);
-		uint32_t recBase = 0, genBase = 0, itemBase = 0, idBase = 0;
-		if (EMIT)
-		{
-			recBase = A.recOff[g];
-			genBase = A.genOff[g];
-			itemBase = A.itemOff[g];
-			idBase = A.recOff[W.primBase];
-		}
-		uint32_t nRec = 0, nGen = 0, nItems = 0;
-		assembleClipped<EMIT>(A, W, slot, genBase, recBase, idBase, itemBase, nRec, nGen, nItems);
-		if (!EMIT)
-		{
-			A.recOff[g] = nRec;
-			A.genOff[g] = nGen;
-			A.itemOff[g] = nItems;
-		}
+		const uint32_t i = base + threadIdx.x / CLIP_GROUP;
+		const bool has = i < n;
+		clipGroup<EMIT>(A, has ? A.clipList[i] : 0u, has, gl);
 	}
 }
 
@@ -849,31 +1019,38 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 		return;
 	const uint32_t g = W.primBase + k;   /* slot of this input in the count arrays */
 	const SrpDrawParams* P = W.P;
-	Pool pl;
 	uint64_t s[2];
 	lineStream(W.prim, W.start, k, W.count, s);
+	uint32_t slot[2];
 	for (int i = 0; i < 2; i++)
+		slot[i] = slotOf(W, s[i]);
+	if (!slotsValid(A, P, slot, 2))
 	{
-		pl.slot[i] = slotOf(W, s[i]);
-		loadClip(P, pl.slot[i], pl.c[i]);
+		if (!EMIT)
+		{
+			A.recOff[g] = 0;
+			A.genOff[g] = 0;
+			A.itemOff[g] = 0;
+		}
+		return;
 	}
-	pl.slot[2] = pl.slot[1];
-	pl.n = 2;
+	float c[2][4];
+	for (int i = 0; i < 2; i++)
+		loadClip(P, slot[i], c[i]);
 
-	/* Liang-Barsky style clip: both endpoints re-derived from the originals. */
-	int ia = 0, ib = 1;
+	/* Liang-Barsky style clip (clipping.c:139-183): each clipped end is
+	 * re-derived from the two originals at t0 / t1. */
+	float t0 = 0.0f, t1 = 1.0f;
 	bool rejected = false;
-	const uint32_t c0 = outcode(pl.c[0]), c1 = outcode(pl.c[1]);
+	const uint32_t c0 = outcode(c[0]), c1 = outcode(c[1]);
 	if ((c0 | c1) != 0)
 	{
 		if ((c0 & c1) != 0)
 			rejected = true;
 		else
-		{
-			float t0 = 0.0f, t1 = 1.0f;
 			for (int p = 0; p < 6 && !rejected; p++)
 			{
-				const float da = planeDist(pl.c[0], p), db = planeDist(pl.c[1], p);
+				const float da = planeDist(c[0], p), db = planeDist(c[1], p);
 				if (da < 0.0f && db < 0.0f) { rejected = true; break; }
 				if (da < 0.0f || db < 0.0f)
 				{
@@ -884,14 +1061,6 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 					if (t0 > t1) rejected = true;
 				}
 			}
-			if (!rejected)
-			{
-				/* pool slot 2 is unused for lines; generated vertices start at 3 */
-				pl.n = 3;
-				if (t0 > 0.0f) ia = poolLerp(pl, 0, 1, t0);
-				if (t1 < 1.0f) ib = poolLerp(pl, 0, 1, t1);
-			}
-		}
 	}
 
 	uint32_t nRec = 0, nGen = 0, nItems = 0;
@@ -902,20 +1071,26 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 		const uint32_t itemBase = EMIT ? A.itemOff[g] : 0u;
 		const uint32_t idBase = EMIT ? A.recOff[W.primBase] : 0u;
 		Vtx e[2];
-		const int q[2] = {ia, ib};
+		const bool gen[2] = {t0 > 0.0f, t1 < 1.0f};
+		const float tt[2] = {t0, t1};
 		for (int i = 0; i < 2; i++)
 		{
-			for (int c = 0; c < 4; c++) e[i].c[c] = pl.c[q[i]][c];
-			e[i].invW = divide(e[i].c);
-			if (q[i] < 3)
-				e[i].vref = pl.slot[q[i]];
-			else
+			if (gen[i])
 			{
+				/* clipping.c:244-258 */
+				for (int q = 0; q < 4; q++) e[i].c[q] = c[0][q] * (1 - tt[i]) + c[1][q] * tt[i];
 				e[i].vref = P->cacheSize + genBase + nGen;
 				if (EMIT)
-					materialize(P, pl, q[i], A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
+					lerpVaryings(P, srp::dev::varyingPtr(P, slot[0]), srp::dev::varyingPtr(P, slot[1]), tt[i],
+					             A.genVaryings + (size_t) (genBase + nGen) * P->varyingsSize);
 				nGen++;
 			}
+			else
+			{
+				for (int q = 0; q < 4; q++) e[i].c[q] = c[i][q];
+				e[i].vref = slot[i];
+			}
+			e[i].invW = divide(e[i].c);
 		}
 		SrpLine L;
 		setupLine(P, e[0], e[1], recBase - idBase, L);
@@ -947,6 +1122,16 @@ __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 	const uint32_t g = W.primBase + k;   /* slot of this input in the count arrays */
 	const SrpDrawParams* P = W.P;
 	const uint32_t slot = slotOf(W, W.start + k);
+	if (!slotsValid(A, P, &slot, 1))
+	{
+		if (!EMIT)
+		{
+			A.recOff[g] = 0;
+			A.genOff[g] = 0;
+			A.itemOff[g] = 0;
+		}
+		return;
+	}
 	float c[4];
 	loadClip(P, slot, c);
 	const float x = c[0], y = c[1], z = c[2], w = c[3];
@@ -975,7 +1160,47 @@ __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 	}
 }
 
-void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped)
+/* The tile pairs of the spans listed by the emitting pass: one wave per
+ * span at a time, lanes over its tiles (consecutive items, so the writes
+ * coalesce).  The pairs equal what a per-tile loop in the emitting thread
+ * would write: item item + k is tile (k % nx, k / nx) of the box. */
+__global__ __launch_bounds__(256) void k_expand_spans(const SrpSpan* spans, const uint32_t* count, uint32_t cap,
+                                                      uint32_t* pairKeys, uint32_t* pairVals, uint32_t* pairRec,
+                                                      unsigned long long* pairMask, uint32_t itemCap)
+{
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+	const uint32_t nWaves = gridDim.x * 4u;
+	uint32_t n = *count;
+	if (n > cap)
+		n = cap;
+	for (uint32_t i = wave; i < n; i += nWaves)
+	{
+		const SrpSpan sp = spans[i];
+		const uint32_t total = sp.nx * sp.ny;
+		const uint64_t m = (sp.rec == 0xFFFFFFFFu) ? 0ull : ~0ull;
+		for (uint32_t k = lane; k < total; k += 64u)
+		{
+			const uint32_t o = sp.item + k;
+			if (o >= itemCap)
+				break;
+			const uint32_t dy = k / sp.nx, dx = k - dy * sp.nx;
+			pairKeys[o] = sp.keyBase + dy * sp.stride + dx;
+			pairVals[o] = o;
+			pairRec[o] = sp.rec;
+			pairMask[o] = m;
+		}
+	}
+}
+
+void launchExpandSpans(const AsmArgs& A, uint32_t blocks, hipStream_t s)
+{
+	if (blocks)
+		hipLaunchKernelGGL(k_expand_spans, dim3(blocks), dim3(256), 0, s, A.spans, A.spanCount, A.spanCap, A.pairKeys,
+		                   A.pairVals, A.pairRec, A.pairMask, A.itemCap);
+}
+
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped, uint32_t clipHint)
 {
 	const uint32_t blocks = A.nBlocks;
 	if (!blocks) return;
@@ -984,21 +1209,25 @@ void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32
 		case 0:
 		{
 			/* The clip list's length is on the device; a grid strides over
-			 * it.  Each clipped triangle is a long serial job, so a known
-			 * count gets one thread per triangle (up to 1024 blocks); an
-			 * unknown one gets 256 blocks, a compromise between a frame that
-			 * clips tens of thousands of triangles and one that clips none
-			 * (the empty grid still costs its dispatch: 1024 blocks 8.8 us,
-			 * 128 blocks 4.6 us, on the 1080p mesh). */
+			 * it, 8 triangles per block (groups of CLIP_GROUP lanes).  A
+			 * known count gets a group per triangle (up to 2048 blocks); an
+			 * unknown one is sized from the caller's hint (the count of a
+			 * similar earlier draw, a quarter more), at least 256 blocks
+			 * (the empty grid of a draw that clips nothing still costs its
+			 * dispatch), and never more blocks than the draw's main
+			 * assembly grid. */
 			uint32_t clipBlocks;
 			if (clipped != kClipUnknown)
 			{
-				const uint32_t need = (uint32_t) (((uint64_t) clipped + 127u) / 128u);
-				clipBlocks = need < 1024u ? need : 1024u;
+				const uint32_t need = (uint32_t) (((uint64_t) clipped + 7u) / 8u);
+				clipBlocks = need < 2048u ? need : 2048u;
 			}
 			else
-				clipBlocks = 256u;
-			clipBlocks = clipBlocks < blocks ? clipBlocks : blocks;
+			{
+				const uint64_t want = ((uint64_t) clipHint + clipHint / 4u + 7u) / 8u;
+				clipBlocks = (uint32_t) (want < 256u ? 256u : (want > 2048u ? 2048u : want));
+				clipBlocks = clipBlocks < blocks ? clipBlocks : blocks;
+			}
 			if (emit)
 			{
 				hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);

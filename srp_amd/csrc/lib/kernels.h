@@ -10,7 +10,7 @@
 #include "srp/device/abi.h"
 #include "srp/device/pipeline.hpp"
 
-#define SRP_POOL 32       /* clip pool: 3 originals + generated vertices */
+#define SRP_POOL 32       /* vertices one triangle's clip may hold: 3 originals + up to 29 generated */
 #define SRP_MAX_POLY 16   /* vertices of a clipped polygon */
 #define SRP_SMALL_BBOX 24 /* largest box side (P->maskBox) for exact tile masks at binning */
 
@@ -35,6 +35,18 @@ struct AsmDraw
 	uint32_t firstBlock;  /* first block of the draw in the launch */
 };
 
+/* A triangle binned to every tile of its box (no exact masks): its tile
+ * pairs are items item .. item + nx*ny - 1, tile (dx, dy) of the box at
+ * item + dy*nx + dx with key keyBase + dy*stride + dx (k_expand_spans). */
+struct SrpSpan
+{
+	uint32_t item;
+	uint32_t rec;         /* triangle record, 0xFFFFFFFF: none (capacity exceeded) */
+	uint32_t keyBase;     /* tile key of the box's first tile */
+	uint32_t stride;      /* tiles per framebuffer row */
+	uint32_t nx, ny;      /* box size in tiles */
+};
+
 /* Arguments of the assembly kernels (assemble.hip). */
 struct AsmArgs
 {
@@ -56,11 +68,17 @@ struct AsmArgs
 	SrpFrag* frags;
 	uint32_t* fragKeys;
 	uint8_t* genVaryings;
-	uint32_t* error;      /* bit 0: clip overflow; bit 1: a capacity below was exceeded */
+	uint32_t* error;      /* bit 0: clip overflow; bit 1: a capacity below was exceeded;
+	                         bit 2: a vertex index outside the prepared cache range */
 	/* triangles (input slots) that need clipping, appended by the counting
 	 * pass; the counter starts at zero and has room for every input */
 	uint32_t* clipList;
 	uint32_t* clipCount;
+	/* spans: the counting pass counts them into *spanCount; the emitting
+	 * pass appends them (from a zeroed *spanCount) for k_expand_spans */
+	SrpSpan* spans;
+	uint32_t* spanCount;
+	uint32_t spanCap;
 	/* capacities of the triangle outputs (records, tile pairs, generated
 	 * vertices); what lies beyond them is dropped and flagged */
 	uint32_t recCap, itemCap, genCap;
@@ -68,21 +86,33 @@ struct AsmArgs
 
 /* kind: 0 triangles, 1 lines, 2 points.  clipped: how many triangles the
  * counting pass listed for clipping, when the caller knows it (0 skips the
- * clipped-triangle kernel); kClipUnknown sizes its grid for the worst case. */
+ * clipped-triangle kernel); with kClipUnknown its grid is sized from
+ * clipHint (an expected count; any value is correct, the grid strides). */
 constexpr uint32_t kClipUnknown = 0xFFFFFFFFu;
-void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped = kClipUnknown);
+void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped = kClipUnknown,
+                    uint32_t clipHint = 0);
+/* After the emitting pass of filled triangles: write the spans' pairs
+ * (`blocks` blocks of 4 waves stride over the device-side span count). */
+void launchExpandSpans(const AsmArgs& A, uint32_t blocks, hipStream_t s);
 
 /* util.hip */
 struct ScanScratch
 {
 	uint32_t* buf = nullptr;   /* block sums of every level */
 	size_t cap = 0;
+	unsigned long long* buf64 = nullptr;   /* exact block sums (multi-array scan) */
+	size_t cap64 = 0;
 };
 /* Exclusive prefix sum of n u32 values in place; *total (device) = sum. */
 void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scratch, hipStream_t s);
-/* The same for nArrays arrays of n values `stride` apart; totals[a] = sums. */
+/* The same for nArrays arrays of n values `stride` apart; totals[a] = sums
+ * (mod 2^32).  With totals64, totals64[a] receives the exact sums: the
+ * scanned offsets are valid only while those fit 32 bits. */
 void scanExclusiveMulti(uint32_t* base, size_t stride, int nArrays, size_t n, uint32_t* totals,
-                        ScanScratch& scratch, hipStream_t s);
+                        ScanScratch& scratch, hipStream_t s, unsigned long long* totals64 = nullptr);
+/* Allocate now the scratch a multi-array scan of nArrays x n values (or a
+ * radix sort of n items) needs, so that later calls allocate nothing. */
+void scanReserve(ScanScratch& scratch, size_t n, int nArrays);
 /* [start, end) of each tile's run in a tile-sorted key array.  With nDev,
  * n is a capacity and the count is min(n, *nDev), read on the device. */
 void tileBounds(const uint32_t* keys, size_t n, uint32_t* start, uint32_t* end, hipStream_t s,
@@ -102,8 +132,11 @@ struct SortScratch
  * is min(n, *nDev), read on the device (no host synchronisation). */
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& scratch,
                uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr);
+void radixSortReserve(SortScratch& scratch, size_t n);
 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+/* *dst = *src by a one-thread kernel (dst may be host-mapped memory) */
+void copyU32(const uint32_t* src, uint32_t* dst, hipStream_t s);
 void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void iota(uint32_t* p, size_t n, hipStream_t s);
 void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv, uint8_t* stencil, uint8_t sv, size_t n,

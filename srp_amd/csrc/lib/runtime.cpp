@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #define SRP_INCLUDE_VEC
@@ -27,7 +28,9 @@
 #include "srp/srp_hip.h"
 #include "srp/device/abi.h"
 #include "kernels.h"
+#include "sizing.h"
 #include "../host/png_decode.h"
+#include "../host/png_encode.h"
 
 using namespace srp::lib;
 
@@ -95,10 +98,12 @@ struct DevBuf
 /* Parameter block of a draw: SrpDrawParams, the draw's assembly record,
  * then (256-aligned) the inline uniform. */
 constexpr size_t kAsmDrawOff = (sizeof(SrpDrawParams) + 63) & ~(size_t) 63;
-/* the draw's eight device counters (assembly totals, error flags, clip
- * count) ride in the block's padding, so the parameter upload zeroes them */
+/* the draw's device counters ride in the block's padding, so the parameter
+ * upload zeroes them: eight u32 (assembly totals mod 2^32, error flags, clip
+ * count), then the three exact u64 totals of the counting pass */
 constexpr size_t kTotalsOff = (kAsmDrawOff + sizeof(AsmDraw) + 15) & ~(size_t) 15;
-constexpr size_t kParamsHead = (kTotalsOff + 8 * sizeof(uint32_t) + 255) & ~(size_t) 255;
+constexpr size_t kTotalsBytes = 8 * sizeof(uint32_t) + 4 * sizeof(uint64_t);
+constexpr size_t kParamsHead = (kTotalsOff + kTotalsBytes + 255) & ~(size_t) 255;
 constexpr size_t kMaxInlineUniform = 16384;
 constexpr size_t kStatWords = (size_t) SRP_STAT_SLOTS * SRP_STAT_STRIDE;
 constexpr size_t kParamsBlock = kParamsHead + kMaxInlineUniform;
@@ -110,16 +115,18 @@ struct Runtime
 	SrpDrawParams* dParams = nullptr;
 	uint8_t* hUniform = nullptr;        /* pinned staging */
 	size_t uniformCap = 0;
-	uint32_t* hTotals = nullptr;        /* pinned: rec, gen, items, error */
+	uint32_t* hTotals = nullptr;        /* pinned: the draw's counters (kTotalsBytes) */
 	unsigned long long* dStats = nullptr;
 	uint64_t hostPrims = 0, hostDraws = 0;
 	int tile = 8;
 	int batched = 1;  /* three-phase batched rasteriser (tile 8) */
 	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
 	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
+	uint32_t depthCull = 1; /* tile-level depth culling in runs */
+	uint32_t lastClipped = 0;   /* triangles the previous draw clipped (grid hint) */
 
 	DevBuf uniform, clip, varyings, genVaryings, counts;
-	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount;
+	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount, spans;
 	ScanScratch scan;
 	SortScratch sort;
 
@@ -166,7 +173,7 @@ struct Runtime
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 		HIP_CHECK(hipHostMalloc((void**) &hParams, kParamsBlock, hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dParams, kParamsBlock));
-		HIP_CHECK(hipHostMalloc((void**) &hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
+		HIP_CHECK(hipHostMalloc((void**) &hTotals, kTotalsBytes, hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dStats, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipMemset(dStats, 0, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
@@ -335,6 +342,7 @@ struct SRPVertexBuffer
 {
 	size_t nBytesPerVertex = 0, nVertices = 0, nBytesAllocated = 0;
 	uint8_t* dData = nullptr;
+	uint64_t version = 0;   /* bumped by every CopyData (batches re-prepare) */
 };
 
 struct SRPIndexBuffer
@@ -345,11 +353,20 @@ struct SRPIndexBuffer
 	std::vector<uint8_t> host;
 	/* min/max vertex index per block of 4096 indices, for range queries */
 	std::vector<uint64_t> blkMin, blkMax;
+	uint64_t version = 0;   /* bumped by every CopyData (batches re-prepare) */
 };
 
 namespace {
 
 constexpr size_t kIdxBlock = 4096;
+
+/* Buffers and framebuffers not yet freed (a batch checks the ones it uses
+ * before each frame). */
+std::unordered_set<const void*>& liveObjects()
+{
+	static auto* s = new std::unordered_set<const void*>();
+	return *s;
+}
 
 uint64_t hostIndex(const SRPIndexBuffer* b, size_t i)
 {
@@ -480,11 +497,17 @@ void srpDepthCompareOp(SRPCompareOp op) { ctx()->depth.compareOp = op; }
 
 /* ---------------------------------------------------------------- buffers */
 
-SRPVertexBuffer* srpNewVertexBuffer(void) { return new SRPVertexBuffer(); }
+SRPVertexBuffer* srpNewVertexBuffer(void)
+{
+	SRPVertexBuffer* b = new SRPVertexBuffer();
+	liveObjects().insert(b);
+	return b;
+}
 
 void srpFreeVertexBuffer(SRPVertexBuffer* b)
 {
 	if (!b) return;
+	liveObjects().erase(b);
 	if (b->dData)
 	{
 		HIP_CHECK(hipStreamSynchronize(rt().stream));
@@ -510,15 +533,22 @@ void srpVertexBufferCopyData(SRPVertexBuffer* b, size_t stride, size_t nBytes, c
 	}
 	b->nBytesPerVertex = stride;
 	b->nVertices = stride ? nBytes / stride : 0;
+	b->version++;
 	HIP_CHECK(hipMemcpyAsync(b->dData, data, nBytes, hipMemcpyHostToDevice, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 }
 
-SRPIndexBuffer* srpNewIndexBuffer(void) { return new SRPIndexBuffer(); }
+SRPIndexBuffer* srpNewIndexBuffer(void)
+{
+	SRPIndexBuffer* b = new SRPIndexBuffer();
+	liveObjects().insert(b);
+	return b;
+}
 
 void srpFreeIndexBuffer(SRPIndexBuffer* b)
 {
 	if (!b) return;
+	liveObjects().erase(b);
 	if (b->dData)
 	{
 		HIP_CHECK(hipStreamSynchronize(rt().stream));
@@ -560,6 +590,7 @@ void srpIndexBufferCopyData(SRPIndexBuffer* b, SRPType type, size_t nBytes, cons
 	}
 	b->type = type;
 	b->elem = sizeofIndexType(type);
+	b->version++;
 	b->nIndices = b->elem ? nBytes / b->elem : 0;
 	b->host.assign((const uint8_t*) data, (const uint8_t*) data + nBytes);
 	const size_t nb = (b->nIndices + kIdxBlock - 1) / kIdxBlock;
@@ -602,12 +633,14 @@ SRPFramebuffer* srpNewFramebuffer(size_t w, size_t h)
 	fillU32(F->dColor, 0, n, R.stream);
 	fillU32((uint32_t*) F->dDepth, 0, n, R.stream);
 	HIP_CHECK(hipStreamSynchronize(R.stream));
+	liveObjects().insert(&F->pub);
 	return &F->pub;
 }
 
 void srpFreeFramebuffer(SRPFramebuffer* fb)
 {
 	if (!fb) return;
+	liveObjects().erase(fb);
 	FbImpl* F = fbImpl(fb);
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
 	HIP_CHECK(hipHostFree(F->pub.color));
@@ -659,6 +692,28 @@ void srpFramebufferDownload(const SRPFramebuffer* fb)
 		return;
 	}
 	fbDownload(fbImpl(fb));
+}
+
+/* tests/utils/save.c:4-33 of the reference (its test harness): the colour
+ * buffer as an RGBA PNG with alpha 255.  A deferred framebuffer's device
+ * copy is downloaded first. */
+int srpFramebufferSaveImage(const SRPFramebuffer* fb, const char* path)
+{
+	if (!fb || !path)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer or path");
+		return -1;
+	}
+	FbImpl* F = fbImpl(fb);
+	if (F->deferred)
+		fbDownload(F);
+	if (fb->width > (1u << 24) || fb->height > (1u << 24) ||
+	    srpPngSaveColorBuffer(path, (int) fb->width, (int) fb->height, fb->color) != 0)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Could not write %s", path);
+		return -1;
+	}
+	return 0;
 }
 
 void srpFramebufferUpload(const SRPFramebuffer* fb)
@@ -888,6 +943,38 @@ void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox)
 	if (pixelRuns >= 0) R.bigPixel = pixelRuns ? 1u : 0u;
 	if (maskBox >= 0) R.maskBox = (uint32_t) (maskBox > SRP_SMALL_BBOX ? SRP_SMALL_BBOX : maskBox);
 }
+
+/* Test hook (not part of the public headers): the multi-array exclusive
+ * scan the draws use, on host data.  counts holds nArrays arrays of n u32;
+ * offsets receives their scans (mod 2^32), totals32 the sums mod 2^32 and
+ * totals64 the exact sums. */
+void srp_lib_scan_exact(const uint32_t* counts, size_t n, int nArrays, uint32_t* offsets, uint32_t* totals32,
+                        uint64_t* totals64)
+{
+	Runtime& R = rt();
+	uint32_t* d = nullptr;
+	uint32_t* t32 = nullptr;
+	unsigned long long* t64 = nullptr;
+	const size_t bytes = n * (size_t) nArrays * sizeof(uint32_t);
+	HIP_CHECK(hipMalloc((void**) &d, bytes + 16));
+	HIP_CHECK(hipMalloc((void**) &t32, 16 * sizeof(uint32_t)));
+	HIP_CHECK(hipMalloc((void**) &t64, 16 * sizeof(unsigned long long)));
+	HIP_CHECK(hipMemcpyAsync(d, counts, bytes, hipMemcpyHostToDevice, R.stream));
+	ScanScratch sc;
+	scanExclusiveMulti(d, n, nArrays, n, t32, sc, R.stream, t64);
+	HIP_CHECK(hipMemcpyAsync(offsets, d, bytes, hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(totals32, t32, (size_t) nArrays * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(totals64, t64, (size_t) nArrays * sizeof(uint64_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	HIP_CHECK(hipFree(d));
+	HIP_CHECK(hipFree(t32));
+	HIP_CHECK(hipFree(t64));
+	if (sc.buf) HIP_CHECK(hipFree(sc.buf));
+	if (sc.buf64) HIP_CHECK(hipFree(sc.buf64));
+}
+
+void srpHipSetTileDepthCull(int enable) { rt().depthCull = enable ? 1u : 0u; }
+int srpHipGetTileDepthCull(void) { return (int) rt().depthCull; }
 
 }  // extern "C"
 
@@ -1120,8 +1207,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.stencil = F->dStencil;
 
 	const size_t usize = vse->uniformSize > fse->uniformSize ? vse->uniformSize : fse->uniformSize;
-	size_t paramsBytes = kTotalsOff + 8 * sizeof(uint32_t);
-	memset((uint8_t*) R.hParams + kTotalsOff, 0, 8 * sizeof(uint32_t));
+	size_t paramsBytes = kTotalsOff + kTotalsBytes;
+	memset((uint8_t*) R.hParams + kTotalsOff, 0, kTotalsBytes);
 	if (sp->uniform && usize)
 	{
 		if (usize <= kMaxInlineUniform)
@@ -1203,17 +1290,35 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	A.clipList = counts + 3 * (nIn + 1);
 	A.error = totals + 3;
 	A.clipCount = totals + 4;
+	A.spanCount = totals + 6;   /* counting pass: how many spans */
+	/* the counting pass's totals are checked (exactly, in 64 bits) before
+	 * the emitting pass, so the capacities can be the totals themselves */
 	A.recCap = A.itemCap = A.genCap = 0xFFFFFFFFu;
-	launchAssemble(A, kind, false, R.stream);
-	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream);
+	unsigned long long* totals64 = reinterpret_cast<unsigned long long*>(totals + 8);
+	launchAssemble(A, kind, false, R.stream, kClipUnknown, R.lastClipped);
+	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream, totals64);
 	if (R.profile) R.span(3, evDraw, R.mark());
-	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
-	const uint32_t nRec = R.hTotals[0], nGen = R.hTotals[1], nItems = R.hTotals[2];
-	const uint32_t nClipped = R.hTotals[4];
+	const uint64_t* hTotals64 = reinterpret_cast<const uint64_t*>(R.hTotals + 8);
 	if (R.hTotals[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
 		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
+	{
+		const DrawTotals dt = {hTotals64[0], hTotals64[1], hTotals64[2], cacheSize};
+		const DrawLimit lim = checkDrawTotals(dt);
+		if (lim != DRAW_FITS)
+		{
+			char text[256];
+			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "%s",
+			        drawLimitText(lim, dt, text, sizeof(text)));
+			return;   /* nothing emitted: the device copy is unchanged */
+		}
+	}
+	const uint32_t nRec = (uint32_t) hTotals64[0], nGen = (uint32_t) hTotals64[1], nItems = (uint32_t) hTotals64[2];
+	const uint32_t nClipped = R.hTotals[4], nSpans = R.hTotals[6];
+	if (kind == 0)
+		R.lastClipped = nClipped;
 	R.hostDraws++;
 	R.hostPrims += nRec;
 	if (nRec == 0)
@@ -1242,6 +1347,9 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		tileStart = R.tileCount.get<uint32_t>(((size_t) 2 * nTiles + 2) * 4);
 		tileEnd = tileStart + nTiles;
 		fillU32(tileStart, 0u, (size_t) 2 * nTiles, R.stream);
+		A.spans = R.spans.get<SrpSpan>((size_t) nSpans * sizeof(SrpSpan) + 64);
+		A.spanCap = nSpans;
+		A.spanCount = totals + 5;   /* emitting pass: append cursor (zeroed by the upload) */
 	}
 	else
 	{
@@ -1251,6 +1359,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		A.fragKeys = R.fragKeys.get<uint32_t>((size_t) nItems * 4 + 16);
 	}
 	launchAssemble(A, kind, true, R.stream, nClipped);
+	if (backend == 0 && nSpans)
+		launchExpandSpans(A, std::min<uint32_t>((nSpans + 3u) / 4u, 2048u), R.stream);
 
 	if (backend == 0)
 	{
@@ -1272,6 +1382,12 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		rl.tile = (uint32_t) R.tile;
 		rl.batched = (R.batched && R.tile == 8) ? 1u : 0u;
 		rl.bigPixel = R.bigPixel;
+		/* culling kernels only where culling can act (the kernel checks the
+		 * same conditions; the other kernels need fewer registers) */
+		const bool monotone = S.depthOp == SRP_COMPARE_LESS || S.depthOp == SRP_COMPARE_LEQUAL ||
+		                      S.depthOp == SRP_COMPARE_GREATER || S.depthOp == SRP_COMPARE_GEQUAL ||
+		                      S.depthOp == SRP_COMPARE_NEVER;
+		rl.depthCull = (R.depthCull && S.depthTest && monotone && !S.stencilEnabled && !S.lateZ) ? 1u : 0u;
 		const uint32_t perBlock = rl.batched ? 1u : ((R.tile == 8) ? 4u : 1u);
 		rl.nBlocks = (nTiles + perBlock - 1) / perBlock;
 		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
@@ -1339,6 +1455,15 @@ void srpDrawIndexBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, con
 
 struct SRPBatch
 {
+	std::vector<SRPDrawCommand> src;         /* every command as given; contexts point into ctxCopy */
+	std::vector<SRPContext> ctxCopy;         /* each draw's render state when the batch was made */
+	/* objects the batch reads: checked live (and unchanged, else the batch
+	 * is prepared again) before each frame */
+	struct Watch { const void* obj; int kind; size_t draw; uint64_t version; };
+	std::vector<Watch> watch;
+	uint32_t* hErr = nullptr;                /* host-mapped: error bits of the last frame */
+	uint32_t* dErr = nullptr;                /* its device address */
+	uint32_t reported = 0;                   /* error bits already reported */
 	std::vector<SRPDrawCommand> cmds;        /* drawn (not culled away) */
 	struct Clear { FbImpl* F; uint32_t depthBits; uint8_t stencil; };
 	std::vector<Clear> clears;
@@ -1362,16 +1487,18 @@ struct SRPBatch
 	uint32_t* totals = nullptr;
 	uint32_t* hTotals = nullptr;             /* pinned */
 	SrpTri* tris = nullptr;
+	SrpSpan* spans = nullptr;                /* recCap entries */
 	uint32_t *pairKeys = nullptr, *pairVals = nullptr, *pairRec = nullptr;
 	unsigned long long* pairMask = nullptr;
 	uint8_t* genVaryings = nullptr;
 	uint32_t recCap = 0, itemCap = 0, genCap = 0;
 	uint32_t nRec = 0, nItems = 0, nGen = 0;
+	uint32_t nClipped = 0;                   /* triangles clipped at sizing (grid hint) */
 	uint32_t* tileRange = nullptr;           /* start[totalTiles], end[totalTiles] */
 	uint32_t* queues = nullptr;
 	ScanScratch scan;
 	SortScratch sort;
-	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1, 1};
+	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1, 1, 1};
 	int cus = 256;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -1414,7 +1541,7 @@ void batchUpload(SRPBatch* B)
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 }
 
-AsmArgs batchAsmArgs(SRPBatch* B)
+AsmArgs batchAsmArgs(SRPBatch* B, bool emit)
 {
 	AsmArgs A;
 	memset(&A, 0, sizeof(A));
@@ -1436,6 +1563,9 @@ AsmArgs batchAsmArgs(SRPBatch* B)
 	A.recCap = B->recCap;
 	A.itemCap = B->itemCap;
 	A.genCap = B->genCap;
+	A.spans = B->spans;
+	A.spanCap = B->recCap;
+	A.spanCount = B->totals + (emit ? 5 : 6);
 	return A;
 }
 
@@ -1448,37 +1578,58 @@ void batchFront(SRPBatch* B, hipStream_t s)
 	vl.nSeg = (uint32_t) B->cmds.size();
 	vl.nBlocks = B->vsBlocks;
 	B->vse->launch(&vl, (void*) s);
-	fillU32(B->totals, 0u, 8, s);
-	AsmArgs A = batchAsmArgs(B);
-	launchAssemble(A, 0, false, s);
-	scanExclusiveMulti(A.recOff, (size_t) B->nIn + 1, 3, B->nIn, B->totals, B->scan, s);
+	fillU32(B->totals, 0u, kTotalsBytes / 4, s);
+	AsmArgs A = batchAsmArgs(B, false);
+	launchAssemble(A, 0, false, s, kClipUnknown, B->nClipped);
+	scanExclusiveMulti(A.recOff, (size_t) B->nIn + 1, 3, B->nIn, B->totals, B->scan, s,
+	                   reinterpret_cast<unsigned long long*>(B->totals + 8));
 }
 
-/* Counting pass on the device, totals to the host; buffers (re)sized. */
-void batchSize(SRPBatch* B)
+/* Counting pass on the device, totals to the host; buffers (re)sized.
+ * False (with a message) when the batch exceeds what one frame can hold. */
+bool batchSize(SRPBatch* B, const char* caller)
 {
 	Runtime& R = rt();
 	batchUpload(B);
 	batchFront(B, R.stream);
-	HIP_CHECK(hipMemcpyAsync(B->hTotals, B->totals, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(B->hTotals, B->totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
-	B->nRec = B->hTotals[0];
-	B->nGen = B->hTotals[1];
-	B->nItems = B->hTotals[2];
+	const uint64_t* t64 = reinterpret_cast<const uint64_t*>(B->hTotals + 8);
 	if (B->hTotals[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
 		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
+	uint64_t slots = 0;
+	for (const SrpSeg& sg : B->hVsSeg)
+		slots += sg.count;
+	/* with headroom for the capacities below */
+	const DrawTotals dt = {t64[0] + 64, t64[1] + 64, t64[2] + 256 + (t64[2] + 256) / 8, slots};
+	const DrawLimit lim = checkDrawTotals(dt);
+	if (lim != DRAW_FITS)
+	{
+		char text[256];
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, caller, "%s", drawLimitText(lim, dt, text, sizeof(text)));
+		return false;
+	}
+	B->nRec = (uint32_t) t64[0];
+	B->nGen = (uint32_t) t64[1];
+	B->nItems = (uint32_t) t64[2];
+	B->nClipped = B->hTotals[4];
 	const uint32_t recCap = B->nRec + 64, genCap = B->nGen + 64, itemCap = B->nItems + 256;
+	/* the scan's and sort's scratch too: frames allocate nothing */
+	scanReserve(B->scan, B->nIn, 3);
 	if (recCap > B->recCap || genCap > B->genCap || itemCap > B->itemCap)
 	{
 		HIP_CHECK(hipDeviceSynchronize());
-		void* bufs[] = {B->tris, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings};
+		void* bufs[] = {B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings};
 		for (void* p : bufs)
 			if (p) HIP_CHECK(hipFree(p));
-		B->recCap = recCap + recCap / 8;
-		B->genCap = genCap + genCap / 8;
-		B->itemCap = itemCap + itemCap / 8;
+		/* headroom for frames whose data grow; never past the per-call limits */
+		auto grow = [](uint64_t c, uint64_t lim) { return (uint32_t) std::min<uint64_t>(c + c / 8, lim); };
+		B->recCap = grow(recCap, kMaxDrawRecords);
+		B->genCap = grow(genCap, kMaxDrawVertexSlots);
+		B->itemCap = grow(itemCap, kMaxDrawItems);
 		HIP_CHECK(hipMalloc((void**) &B->tris, (size_t) B->recCap * sizeof(SrpTri)));
+		HIP_CHECK(hipMalloc((void**) &B->spans, (size_t) B->recCap * sizeof(SrpSpan)));
 		HIP_CHECK(hipMalloc((void**) &B->pairKeys, (size_t) B->itemCap * 4));
 		HIP_CHECK(hipMalloc((void**) &B->pairVals, (size_t) B->itemCap * 4));
 		HIP_CHECK(hipMalloc((void**) &B->pairRec, (size_t) B->itemCap * 4));
@@ -1486,6 +1637,8 @@ void batchSize(SRPBatch* B)
 		HIP_CHECK(hipMalloc((void**) &B->genVaryings, (size_t) B->genCap * B->sp->vs->varyingsSize + 512));
 		batchUpload(B);
 	}
+	radixSortReserve(B->sort, B->itemCap);
+	return true;
 }
 
 /* One frame of launches (capturable: fixed grids, no allocation, no host
@@ -1495,8 +1648,9 @@ void batchLaunch(SRPBatch* B, hipStream_t s)
 	for (const SRPBatch::Clear& c : B->clears)
 		fillFramebuffer(c.F->dColor, 0u, (uint32_t*) c.F->dDepth, c.depthBits, c.F->dStencil, c.stencil, c.F->pub.size, s);
 	batchFront(B, s);
-	AsmArgs A = batchAsmArgs(B);
-	launchAssemble(A, 0, true, s);
+	AsmArgs A = batchAsmArgs(B, true);
+	launchAssemble(A, 0, true, s, kClipUnknown, B->nClipped);
+	launchExpandSpans(A, std::min<uint32_t>((B->recCap + 3u) / 4u, 2048u), s);
 	uint32_t* tileStart = B->tileRange;
 	uint32_t* tileEnd = B->tileRange + B->totalTiles;
 	fillU32(B->tileRange, 0u, (size_t) 2 * B->totalTiles, s);
@@ -1519,6 +1673,7 @@ void batchLaunch(SRPBatch* B, hipStream_t s)
 	rl.tile = 8;
 	rl.batched = 1;
 	rl.bigPixel = B->tuning.pixelRuns ? 1u : 0u;
+	rl.depthCull = B->tuning.depthCull ? 1u : 0u;
 	uint32_t waves = (uint32_t) (B->cus * B->tuning.wavesPerCU);
 	if (waves > B->totalTiles) waves = B->totalTiles;
 	rl.nBlocks = waves ? waves : 1u;
@@ -1527,6 +1682,27 @@ void batchLaunch(SRPBatch* B, hipStream_t s)
 	rl.xcdQueues = B->tuning.xcdQueues ? 1u : 0u;
 	rl.queues = B->queues;
 	B->fse->raster(&rl, (void*) s);
+	/* the frame's error bits, where the host reads them without waiting */
+	copyU32(B->totals + 3, B->dErr, s);
+}
+
+/* Report error bits raised by frames since the last report. */
+void batchReport(SRPBatch* B)
+{
+	const uint32_t e = __atomic_load_n(B->hErr, __ATOMIC_ACQUIRE);
+	const uint32_t fresh = e & ~B->reported;
+	if (fresh & 1u)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
+		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
+	if (fresh & 2u)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpBatchRender",
+		        "A frame exceeded the batch's record, tile-entry or clip-vertex capacity; its excess primitives "
+		        "were dropped");
+	if (fresh & 4u)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpBatchRender",
+		        "A frame met vertex indices outside the range the batch was prepared for; those primitives were "
+		        "dropped");
+	B->reported = e;
 }
 
 /* Apply deferred fills recorded on the batch's framebuffers outside it. */
@@ -1540,27 +1716,43 @@ void batchFlushPending(SRPBatch* B)
 
 extern "C" {
 
-SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
+}  // extern "C"
+
+namespace {
+
+/* Validate and prepare a batch of n commands (srpNewBatch; also when a
+ * batch's buffers changed).  Each command's render state is copied (its
+ * context, or the current one), so the batch can be prepared again later
+ * with the state it was made with. */
+SRPBatch* buildBatch(const SRPDrawCommand* cmds, size_t n, const char* caller)
 {
-	if (!cmds || n == 0)
-	{
-		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewBatch", "No draws");
-		return nullptr;
-	}
 	Runtime& R = rt();
 	SRPBatch* B = new SRPBatch();
 	auto fail = [&](const char* fmt, size_t i) {
-		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewBatch", fmt, i);
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, caller, fmt, i);
 		delete B;
 		return (SRPBatch*) nullptr;
 	};
+	B->ctxCopy.resize(n);
+	B->src.assign(cmds, cmds + n);
+	for (size_t i = 0; i < n; i++)
+	{
+		B->ctxCopy[i] = *(cmds[i].context ? cmds[i].context : ctx());
+		B->src[i].context = &B->ctxCopy[i];
+	}
 	struct Prep { uint64_t lo; uint32_t cacheSize, nIn; const SRPContext* C; };
 	std::vector<Prep> prep;
 	for (size_t i = 0; i < n; i++)
 	{
-		const SRPDrawCommand& c = cmds[i];
+		const SRPDrawCommand& c = B->src[i];
 		if (!c.vb || !c.fb || !c.sp || !c.sp->vs || !c.sp->fs)
 			return fail("Draw %zu: missing vertex buffer, framebuffer or shader program", i);
+		if (!liveObjects().count(c.vb) || !liveObjects().count(c.fb) || (c.ib && !liveObjects().count(c.ib)))
+			return fail("Draw %zu: a buffer or framebuffer it names was freed", i);
+		B->watch.push_back({c.vb, 0, i, c.vb->version});
+		if (c.ib)
+			B->watch.push_back({c.ib, 1, i, c.ib->version});
+		B->watch.push_back({c.fb, 2, i, 0});
 		FbImpl* F = fbImpl(c.fb);
 		if (!F->deferred)
 			return fail("Draw %zu: batches draw into deferred framebuffers (srpFramebufferSetDeferred)", i);
@@ -1630,8 +1822,11 @@ SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
 	const size_t nd = B->cmds.size();
 	B->usize = B->vse->uniformSize > B->fse->uniformSize ? B->vse->uniformSize : B->fse->uniformSize;
 	B->stride = kParamsHead + ((B->usize + 255) & ~(size_t) 255);
-	HIP_CHECK(hipHostMalloc((void**) &B->hTotals, 8 * sizeof(uint32_t), hipHostMallocDefault));
-	HIP_CHECK(hipMalloc((void**) &B->totals, 8 * sizeof(uint32_t)));
+	HIP_CHECK(hipHostMalloc((void**) &B->hTotals, kTotalsBytes, hipHostMallocDefault));
+	HIP_CHECK(hipMalloc((void**) &B->totals, kTotalsBytes));
+	HIP_CHECK(hipHostMalloc((void**) &B->hErr, sizeof(uint32_t), hipHostMallocMapped));
+	*B->hErr = 0;
+	HIP_CHECK(hipHostGetDevicePointer((void**) &B->dErr, B->hErr, 0));
 	if (nd == 0)
 		return B;   /* only clears */
 	HIP_CHECK(hipHostMalloc((void**) &B->hBlocks, nd * B->stride, hipHostMallocDefault));
@@ -1717,12 +1912,65 @@ SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
 	HIP_CHECK(hipMalloc((void**) &B->dRasterSeg, nd * sizeof(SrpSeg)));
 	HIP_CHECK(hipMalloc((void**) &B->tileRange, ((size_t) 2 * B->totalTiles + 2) * sizeof(uint32_t)));
 	HIP_CHECK(hipMalloc((void**) &B->queues, 8 * sizeof(uint32_t)));
-	batchSize(B);
-	/* one eager frame sizes the scan and sort scratch (none is allocated later) */
-	batchFlushPending(B);
-	batchLaunch(B, R.stream);
-	HIP_CHECK(hipStreamSynchronize(R.stream));
+	if (!batchSize(B, caller))
+	{
+		srpFreeBatch(B);
+		return nullptr;
+	}
 	return B;
+}
+
+/* Before a frame: the objects the batch reads must still exist, and a
+ * batch whose vertex or index buffers were given new data (which may move
+ * them, or change the vertex range the draws need) is prepared again from
+ * its commands and their recorded state, keeping its tuning and its graph
+ * (captured again).  False, with a message, when the frame cannot run. */
+bool batchValidate(SRPBatch* B, const char* caller)
+{
+	bool changed = false;
+	for (const SRPBatch::Watch& w : B->watch)
+	{
+		if (!liveObjects().count(w.obj))
+		{
+			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, caller,
+			        "Draw %zu: its %s was freed after the batch was made; nothing is drawn", w.draw,
+			        w.kind == 0 ? "vertex buffer" : (w.kind == 1 ? "index buffer" : "framebuffer"));
+			return false;
+		}
+		if (w.kind == 0 && ((const SRPVertexBuffer*) w.obj)->version != w.version)
+			changed = true;
+		if (w.kind == 1 && ((const SRPIndexBuffer*) w.obj)->version != w.version)
+			changed = true;
+	}
+	if (!changed)
+		return true;
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+	const bool captured = B->exec != nullptr;
+	SRPBatch* fresh = buildBatch(B->src.data(), B->src.size(), caller);
+	if (!fresh)
+		return false;
+	if (memcmp(&fresh->tuning, &B->tuning, sizeof(SRPBatchTuning)) != 0)
+		srpBatchSetTuning(fresh, &B->tuning);
+	fresh->reported = B->reported;
+	std::swap(*B, *fresh);
+	srpFreeBatch(fresh);   /* the old preparation */
+	if (captured && srpBatchCapture(B) != 0)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, caller, "The batch's graph could not be captured again");
+	return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
+{
+	if (!cmds || n == 0)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewBatch", "No draws");
+		return nullptr;
+	}
+	return buildBatch(cmds, n, "srpNewBatch");
 }
 
 void srpFreeBatch(SRPBatch* B)
@@ -1732,10 +1980,16 @@ void srpFreeBatch(SRPBatch* B)
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
 	batchDropGraph(B);
 	void* bufs[] = {B->dBlocks, B->dDraws, B->dVsSeg, B->dRasterSeg, B->clip, B->varyings, B->counts, B->totals,
-	                B->tris, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings, B->tileRange,
-	                B->queues, B->scan.buf, B->sort.keys, B->sort.vals, B->sort.hist, B->sort.scan.buf};
+	                B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings, B->tileRange,
+	                B->queues, B->scan.buf, B->scan.buf64, B->sort.keys, B->sort.vals, B->sort.hist,
+	                B->sort.scan.buf, B->sort.scan.buf64};
 	for (void* p : bufs)
 		batchFreeDev(p);
+	if (B->hErr)
+	{
+		batchReport(B);
+		HIP_CHECK(hipHostFree(B->hErr));
+	}
 	if (B->hBlocks) HIP_CHECK(hipHostFree(B->hBlocks));
 	if (B->hTotals) HIP_CHECK(hipHostFree(B->hTotals));
 	delete B;
@@ -1746,6 +2000,9 @@ void srpBatchRender(SRPBatch* B)
 	if (!B)
 		return;
 	Runtime& R = rt();
+	batchReport(B);
+	if (!batchValidate(B, "srpBatchRender"))
+		return;
 	batchFlushPending(B);
 	if (B->exec)
 	{
@@ -1769,10 +2026,13 @@ int srpBatchCapture(SRPBatch* B)
 	if (!B || B->cmds.empty())
 		return -1;
 	Runtime& R = rt();
+	batchReport(B);
 	batchDropGraph(B);
+	if (!batchValidate(B, "srpBatchCapture"))
+		return -1;
 	batchFlushPending(B);
-	/* an eager frame first: every scratch buffer has its final size */
-	batchLaunch(B, R.stream);
+	/* every scratch buffer was sized when the batch was prepared, so the
+	 * captured frame allocates nothing; nothing is drawn now */
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 	if (hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
 		return -2;
@@ -1807,7 +2067,7 @@ void srpBatchSetTuning(SRPBatch* B, const SRPBatchTuning* t)
 	if (!B->cmds.empty())
 	{
 		if (resize)
-			batchSize(B);   /* the mask box changes the number of tile pairs */
+			batchSize(B, "srpBatchSetTuning");   /* the mask box changes the number of tile pairs */
 		else
 			batchUpload(B);
 	}
@@ -1822,6 +2082,9 @@ void srpBatchGetTuning(const SRPBatch* B, SRPBatchTuning* t)
 double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 {
 	if (!B || B->cmds.empty())
+		return 0.0;
+	batchReport(B);
+	if (!batchValidate(B, "srpBatchAutotune"))
 		return 0.0;
 	if (trials < 1) trials = 1;
 	Runtime& R = rt();
@@ -1839,7 +2102,7 @@ double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 			for (int q : queues)
 				for (int pr : runs)
 			{
-				SRPBatchTuning t = {w, box, q, pr};
+				SRPBatchTuning t = {w, box, q, pr, B->tuning.depthCull};
 				srpBatchSetTuning(B, &t);
 				batchFlushPending(B);
 				batchLaunch(B, R.stream);   /* warm-up */
@@ -1869,6 +2132,7 @@ void srpBatchInfo(const SRPBatch* B, uint64_t out[4])
 {
 	if (!B || !out)
 		return;
+	batchReport(const_cast<SRPBatch*>(B));
 	out[0] = B->cmds.size();
 	out[1] = B->nRec;
 	out[2] = B->nItems;

@@ -58,6 +58,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* data, size_
 
 __global__ void k_copy1(const uint32_t* src, uint32_t* dst) { *dst = *src; }
 
+void copyU32(const uint32_t* src, uint32_t* dst, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, s, src, dst);
+}
+
 static void scanLevel(uint32_t* data, size_t n, uint32_t* total, uint32_t* scratch, hipStream_t s)
 {
 	const size_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
@@ -98,21 +103,30 @@ void scanExclusive(uint32_t* data, size_t n, uint32_t* total, ScanScratch& scrat
  * is cheaper than a third launch while there are at most a few thousand
  * blocks).  The last block writes the array's grand total. */
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(uint32_t* base, size_t stride, size_t n, uint32_t* sums,
-                                                              uint32_t nb)
+                                                              unsigned long long* sums64, uint32_t nb)
 {
 	uint32_t* data = base + (size_t) blockIdx.y * stride;
 	__shared__ uint32_t sh[SCAN_THREADS];
+	__shared__ unsigned long long waveSum[SCAN_THREADS / 64];
 	const uint32_t t = threadIdx.x;
 	const size_t b0 = (size_t) blockIdx.x * SCAN_CHUNK + (size_t) t * SCAN_ITEMS;
 	uint32_t v[SCAN_ITEMS];
 	uint32_t sum = 0;
+	unsigned long long wide = 0;
 #pragma unroll
 	for (int j = 0; j < SCAN_ITEMS; j++)
 	{
 		const size_t i = b0 + j;
 		v[j] = (i < n) ? data[i] : 0u;
 		sum += v[j];
+		wide += v[j];
 	}
+	/* The block's sum without wrap-around: the offsets below are exact only
+	 * while the array's total fits 32 bits, which the caller checks. */
+	for (int off = 32; off > 0; off >>= 1)
+		wide += __shfl_xor(wide, off);
+	if ((t & 63u) == 0)
+		waveSum[t >> 6] = wide;
 	sh[t] = sum;
 	__syncthreads();
 	for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1)
@@ -131,11 +145,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(uint32_t* base, si
 		run += v[j];
 	}
 	if (t == SCAN_THREADS - 1)
+	{
 		sums[(size_t) blockIdx.y * nb + blockIdx.x] = sh[t];
+		unsigned long long w = 0;
+		for (int k = 0; k < SCAN_THREADS / 64; k++) w += waveSum[k];
+		sums64[(size_t) blockIdx.y * nb + blockIdx.x] = w;
+	}
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_finish(uint32_t* base, size_t stride, size_t n,
-                                                              const uint32_t* sums, uint32_t nb, uint32_t* totals)
+                                                              const uint32_t* sums, const unsigned long long* sums64,
+                                                              uint32_t nb, uint32_t* totals,
+                                                              unsigned long long* totals64)
 {
 	uint32_t* data = base + (size_t) blockIdx.y * stride;
 	const uint32_t* s = sums + (size_t) blockIdx.y * nb;
@@ -158,35 +179,74 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_finish(uint32_t* base, si
 			const size_t i = b0 + (size_t) j * SCAN_THREADS + t;
 			if (i < n) data[i] += add;
 		}
-	if (blockIdx.x == nb - 1 && t == 0)
-		totals[blockIdx.y] = add + s[blockIdx.x];
+	if (blockIdx.x == nb - 1)
+	{
+		if (t == 0)
+			totals[blockIdx.y] = add + s[blockIdx.x];
+		if (totals64)
+		{
+			/* the array's exact total (the last block reduces every block's) */
+			__shared__ unsigned long long red64[SCAN_THREADS];
+			const unsigned long long* s64 = sums64 + (size_t) blockIdx.y * nb;
+			unsigned long long acc64 = 0;
+			for (uint32_t k = t; k < nb; k += SCAN_THREADS) acc64 += s64[k];
+			red64[t] = acc64;
+			__syncthreads();
+			for (uint32_t off = SCAN_THREADS / 2; off > 0; off >>= 1)
+			{
+				if (t < off) red64[t] += red64[t + off];
+				__syncthreads();
+			}
+			if (t == 0)
+				totals64[blockIdx.y] = red64[0];
+		}
+	}
+}
+
+/* Exact (64-bit) sum of n u32 values added into *out (zeroed by the caller). */
+__global__ __launch_bounds__(SCAN_THREADS) void k_sum64(const uint32_t* data, size_t n, unsigned long long* out)
+{
+	unsigned long long acc = 0;
+	for (size_t i = (size_t) blockIdx.x * SCAN_THREADS + threadIdx.x; i < n; i += (size_t) gridDim.x * SCAN_THREADS)
+		acc += data[i];
+	for (int off = 32; off > 0; off >>= 1)
+		acc += __shfl_xor(acc, off);
+	if ((threadIdx.x & 63u) == 0 && acc)
+		atomicAdd(out, acc);
 }
 
 void scanExclusiveMulti(uint32_t* base, size_t stride, int nArrays, size_t n, uint32_t* totals, ScanScratch& scratch,
-                        hipStream_t s)
+                        hipStream_t s, unsigned long long* totals64)
 {
 	if (n == 0)
 	{
 		fillU32(totals, 0u, (size_t) nArrays, s);
+		if (totals64)
+			fillU32(reinterpret_cast<uint32_t*>(totals64), 0u, (size_t) nArrays * 2, s);
 		return;
 	}
 	const size_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
 	if (nb > 8192)
 	{
+		if (totals64)
+		{
+			fillU32(reinterpret_cast<uint32_t*>(totals64), 0u, (size_t) nArrays * 2, s);
+			const size_t blocks = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+			const uint32_t g = (uint32_t) (blocks < 2048 ? blocks : 2048);
+			for (int a = 0; a < nArrays; a++)
+				hipLaunchKernelGGL(k_sum64, dim3(g), dim3(SCAN_THREADS), 0, s, base + (size_t) a * stride, n,
+				                   totals64 + a);
+		}
 		for (int a = 0; a < nArrays; a++)
 			scanExclusive(base + (size_t) a * stride, n, totals + a, scratch, s);
 		return;
 	}
-	const size_t need = nb * (size_t) nArrays + 2;
-	if (need > scratch.cap)
-	{
-		if (scratch.buf) hipFree(scratch.buf);
-		scratch.cap = need * 2;
-		hipMalloc(&scratch.buf, scratch.cap * sizeof(uint32_t));
-	}
+	scanReserve(scratch, n, nArrays);
 	const dim3 grid((uint32_t) nb, (uint32_t) nArrays);
-	hipLaunchKernelGGL(k_scan_blocks, grid, dim3(SCAN_THREADS), 0, s, base, stride, n, scratch.buf, (uint32_t) nb);
-	hipLaunchKernelGGL(k_scan_finish, grid, dim3(SCAN_THREADS), 0, s, base, stride, n, scratch.buf, (uint32_t) nb, totals);
+	hipLaunchKernelGGL(k_scan_blocks, grid, dim3(SCAN_THREADS), 0, s, base, stride, n, scratch.buf, scratch.buf64,
+	                   (uint32_t) nb);
+	hipLaunchKernelGGL(k_scan_finish, grid, dim3(SCAN_THREADS), 0, s, base, stride, n, scratch.buf, scratch.buf64,
+	                   (uint32_t) nb, totals, totals64);
 }
 
 /* Tile ranges of a list sorted by tile: [start, end) per tile; tiles that
@@ -368,12 +428,38 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 	}
 }
 
-void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& sc,
-               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev)
+/* Scratch of a multi-array scan of nArrays x n values, allocated now. */
+void scanReserve(ScanScratch& sc, size_t n, int nArrays)
 {
-	*outKeys = keys;
-	*outVals = vals;
-	if (n <= 1 || bits <= 0)
+	if (n == 0)
+		return;
+	const size_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+	size_t need;
+	if (nb > 8192)
+	{
+		need = 2;
+		for (size_t m = n; m > 1; m = (m + SCAN_CHUNK - 1) / SCAN_CHUNK)
+			need += (m + SCAN_CHUNK - 1) / SCAN_CHUNK;
+	}
+	else
+		need = nb * (size_t) nArrays + 2;
+	if (need > sc.cap)
+	{
+		if (sc.buf) hipFree(sc.buf);
+		sc.cap = need * 2;
+		hipMalloc(&sc.buf, sc.cap * sizeof(uint32_t));
+	}
+	if (nb <= 8192 && need > sc.cap64)
+	{
+		if (sc.buf64) hipFree(sc.buf64);
+		sc.cap64 = need * 2;
+		hipMalloc(&sc.buf64, sc.cap64 * sizeof(unsigned long long));
+	}
+}
+
+void radixSortReserve(SortScratch& sc, size_t n)
+{
+	if (n <= 1)
 		return;
 	const uint32_t nBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
 	if (n > sc.cap)
@@ -391,6 +477,20 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 		sc.histCap = (histN + 1) * 2;
 		hipMalloc(&sc.hist, sc.histCap * sizeof(uint32_t));
 	}
+	if (nBlocks > RS_FUSED_MAX_BLOCKS)
+		scanReserve(sc.scan, histN, 1);
+}
+
+void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& sc,
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev)
+{
+	*outKeys = keys;
+	*outVals = vals;
+	if (n <= 1 || bits <= 0)
+		return;
+	const uint32_t nBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
+	radixSortReserve(sc, n);
+	const size_t histN = (size_t) 256 * nBlocks;
 	uint32_t* kIn = keys;
 	uint32_t* vIn = vals;
 	uint32_t* kOut = sc.keys;

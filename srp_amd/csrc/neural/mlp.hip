@@ -605,6 +605,7 @@ void srpFreeNeuralDataset(SRPNeuralDataset* D)
 	if (D->flags) NN_CHECK(hipFree(D->flags));
 	NN_CHECK(hipFree(D->total));
 	if (D->scan.buf) NN_CHECK(hipFree(D->scan.buf));
+	if (D->scan.buf64) NN_CHECK(hipFree(D->scan.buf64));
 	delete D;
 }
 

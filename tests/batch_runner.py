@@ -64,6 +64,14 @@ class BatchScenes:
     def set_tuning(self, waves, box, xcd, runs=1):
         self.lib.srpBenchBatchSetTuning(waves, box, xcd, runs)
 
+    def regenerate(self, k, seed):
+        self.lib.srpBenchBatchRegenerate.argtypes = [ctypes.c_int, ctypes.c_uint64]
+        self.lib.srpBenchBatchRegenerate(k, seed)
+
+    def drop_scene(self, k):
+        self.lib.srpBenchBatchDropScene.argtypes = [ctypes.c_int]
+        self.lib.srpBenchBatchDropScene(k)
+
     def info(self):
         a = (ctypes.c_uint64 * 4)()
         self.lib.srpBenchBatchInfo(ctypes.byref(a))

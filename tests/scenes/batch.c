@@ -294,12 +294,45 @@ double srpBenchBatchAutotune(int trials, int out[4])
 
 void srpBenchBatchSetTuning(int waves, int box, int xcd, int runs)
 {
-	SRPBatchTuning t = {waves, box, xcd, runs};
+	SRPBatchTuning t = {waves, box, xcd, runs, 1};
 	srpBatchSetTuning(g_bb.batch, &t);
 }
 
 void srpBenchBatchInfo(uint64_t out[4]) { srpBatchInfo(g_bb.batch, out); }
+
+/* Free scene k's vertex buffer while the batch still names it (the batch's
+ * frames must then draw nothing and say so); the scene keeps no buffer. */
+void srpBenchBatchDropScene(int k)
+{
+	srpFreeVertexBuffer(g_bb.s[k].vb);
+	g_bb.s[k].vb = NULL;
+}
 #endif
+
+/* New vertex data for scene k: the same number of triangles drawn from
+ * another seed, in a larger allocation (the device buffer moves), so a batch
+ * made earlier must be prepared again before its next frame. */
+void srpBenchBatchRegenerate(int k, uint64_t seed)
+{
+	BScene* sc = &g_bb.s[k];
+	uint64_t s = seed;
+	const size_t nv = 3 * sc->nTri, spare = 3 * sc->nTri + 64;
+	VtxPosColor* v = (VtxPosColor*) calloc(nv + spare, sizeof(VtxPosColor));
+	for (size_t i = 0; i < sc->nTri; i++)
+	{
+		const double cx = 2.0 * bbU(&s) - 1.0, cy = 2.0 * bbU(&s) - 1.0, cz = 2.0 * bbU(&s) - 1.0;
+		const double len = exp(log(0.08) + 0.6 * bbNormal(&s));
+		for (int j = 0; j < 3; j++)
+		{
+			VtxPosColor* p = &v[3 * i + j];
+			p->position = VEC3((float) (cx + len * (2.0 * bbU(&s) - 1.0)), (float) (cy + len * (2.0 * bbU(&s) - 1.0)),
+			                   (float) (cz + len * (2.0 * bbU(&s) - 1.0)));
+			p->color = VEC3((float) bbU(&s), (float) bbU(&s), (float) bbU(&s));
+		}
+	}
+	srpVertexBufferCopyData(sc->vb, sizeof(VtxPosColor), (nv + spare) * sizeof(VtxPosColor), v);
+	free(v);
+}
 
 void srpBenchBatchFinish(void)
 {
@@ -308,7 +341,7 @@ void srpBenchBatchFinish(void)
 #endif
 	for (int k = 0; k < g_bb.n; k++)
 	{
-		srpFreeVertexBuffer(g_bb.s[k].vb);
+		if (g_bb.s[k].vb) srpFreeVertexBuffer(g_bb.s[k].vb);
 		srpFreeFramebuffer(g_bb.s[k].fb);
 	}
 	free(g_bb.s);

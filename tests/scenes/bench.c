@@ -34,10 +34,12 @@ void srpSceneMixDraw(const SRPFramebuffer* fb, const SRPShaderProgram* sp, const
                      const SRPVertexBuffer* vbL, const SRPVertexBuffer* vbP, size_t nTri, size_t nLine, size_t nPoint);
 extern SRPVaryingInfo g_meshVaryings[2];
 extern SRPVaryingInfo g_mixVaryings[3];
+void srpSceneBuildTris(uint64_t seed, size_t nTri, VtxPosColor* v);
+static SRPVaryingInfo g_trisVaryings[1] = {{3, SRP_FLOAT, SRP_INTERPOLATION_MODE_PERSPECTIVE}};
 
 typedef struct Bench
 {
-	int kind;  /* 0 mesh, 1 mix, 2 terrain */
+	int kind;  /* 0 mesh, 1 mix, 2 terrain, 3 random triangles */
 	SRPFramebuffer* fb;       /* current target: fbs[0] or fbs[1] */
 	SRPFramebuffer* fbs[2];
 	SRPVertexBuffer *vb, *vbL, *vbP;
@@ -111,6 +113,29 @@ int srpBenchMixSetup(int w, int h, int nTri, int nLine, int nPoint, uint64_t see
 	g_b.sp.uniform = NULL; g_b.sp.vs = &g_b.vs; g_b.sp.fs = &g_b.fs;
 	benchFbInit(w, h);
 	srpSceneMixState();
+	return 0;
+}
+
+/* Random clip-space triangles (the plain-method workload): nTri
+ * non-indexed triangles, xyz uniform in [-1.2, 1.2]^3 with w = 1, RGB per
+ * vertex, passthrough VS, perspective colour, depth LESS, no culling. */
+int srpBenchTrisSetup(int w, int h, int nTri, uint64_t seed)
+{
+	memset(&g_b, 0, sizeof(g_b));
+	g_b.kind = 3;
+	srpNewContext(&srpContext);
+	VtxPosColor* v = (VtxPosColor*) malloc(3 * (size_t) nTri * sizeof(VtxPosColor));
+	srpSceneBuildTris(seed, (size_t) nTri, v);
+	g_b.vb = srpNewVertexBuffer();
+	srpVertexBufferCopyData(g_b.vb, sizeof(VtxPosColor), 3 * (size_t) nTri * sizeof(VtxPosColor), v);
+	free(v);
+	g_b.nTri = (size_t) nTri;
+	g_b.vs.shader = sc_vsColorPass; g_b.vs.nVaryings = 1; g_b.vs.varyingsInfo = g_trisVaryings; g_b.vs.varyingsSize = sizeof(VarColor);
+	g_b.fs.shader = sc_fsColor; g_b.fs.mayOverwriteDepth = false;
+	g_b.sp.uniform = NULL; g_b.sp.vs = &g_b.vs; g_b.sp.fs = &g_b.fs;
+	benchFbInit(w, h);
+	srpDepthTest(true);
+	srpDepthCompareOp(SRP_COMPARE_LESS);
 	return 0;
 }
 
@@ -231,6 +256,8 @@ void srpBenchFrame(void)
 #endif
 	if (g_b.kind == 0 || g_b.kind == 2)
 		srpDrawIndexBuffer(g_b.ib, g_b.vb, g_b.fb, &g_b.sp, SRP_PRIM_TRIANGLES, 0, g_b.nIdx);
+	else if (g_b.kind == 3)
+		srpDrawVertexBuffer(g_b.vb, g_b.fb, &g_b.sp, SRP_PRIM_TRIANGLES, 0, 3 * g_b.nTri);
 	else
 		srpSceneMixDraw(g_b.fb, &g_b.sp, g_b.vb, g_b.vbL, g_b.vbP, g_b.nTri, g_b.nLine, g_b.nPoint);
 }

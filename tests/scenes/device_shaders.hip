@@ -1,10 +1,7 @@
-/* Device instances of the scene shaders (GPU build of the scene module). */
-#include <srp/srp_device.h>
-#include "shaders.h"
-
-#define HOST_VS(name, fn, usize) SRP_HIP_VERTEX_SHADER(name, fn, usize)
-#define HOST_FS(name, fn, usize) SRP_HIP_FRAGMENT_SHADER(name, fn, usize)
-#include "shader_list.h"
+/* Vertex shaders of shader_list.h (part 0) and the terrain's neural
+ * fragment shader; the other fragment shaders are in device_shaders_1..4.hip. */
+#define SC_DEVICE_PART 0
+#include "device_shaders_part.h"
 
 /* The terrain's neural material (GPU only): the network of
  * srp/device/neural.hpp evaluated on the fragment's surface features; the

@@ -21,8 +21,8 @@
 SRPContext srpContext;
 
 /* Shader symbols (host_shaders.c or device_shaders.hip). */
-#define HOST_VS(name, fn, usize) void name(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
-#define HOST_FS(name, fn, usize) void name(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
+#define HOST_VS(name, fn, usize, part) void name(SRPVertexShaderIn* in, SRPVertexShaderOut* out);
+#define HOST_FS(name, fn, usize, part) void name(SRPFragmentShaderIn* in, SRPFragmentShaderOut* out);
 #include "shader_list.h"
 #undef HOST_VS
 #undef HOST_FS
@@ -433,17 +433,23 @@ static void scDepthInit(SRPFramebuffer* fb, float depth)
 
 /* Random coloured triangles in clip space, xyz ~ U[-1.2, 1.2], w = 1;
  * depth LESS against a +1 depth buffer.  p0 = triangle count. */
-static int scCfgTris(const SceneArgs* a, SceneOut* o)
+/* The random clip-space triangles of the plain-method workload. */
+void srpSceneBuildTris(uint64_t seed, size_t nTri, VtxPosColor* v)
 {
-	const size_t nTri = a->p0 ? (size_t) a->p0 : 2000;
-	uint64_t s = a->seed;
-	VtxPosColor* v = (VtxPosColor*) malloc(3 * nTri * sizeof(VtxPosColor));
+	uint64_t s = seed;
 	for (size_t i = 0; i < 3 * nTri; i++)
 	{
 		const float x = rngF(&s, -1.2f, 1.2f), y = rngF(&s, -1.2f, 1.2f), z = rngF(&s, -1.2f, 1.2f);
 		const float r = rngF(&s, 0, 1), g = rngF(&s, 0, 1), b = rngF(&s, 0, 1);
 		v[i] = pc(x, y, z, r, g, b);
 	}
+}
+
+static int scCfgTris(const SceneArgs* a, SceneOut* o)
+{
+	const size_t nTri = a->p0 ? (size_t) a->p0 : 2000;
+	VtxPosColor* v = (VtxPosColor*) malloc(3 * nTri * sizeof(VtxPosColor));
+	srpSceneBuildTris(a->seed, nTri, v);
 	SRPVertexShader vs = scVS(sc_vsColorPass, 1, g_vColor, sizeof(VarColor));
 	SRPFragmentShader fs = scFS(sc_fsColor, false);
 	SRPShaderProgram sp = scProgram(NULL, &vs, &fs);

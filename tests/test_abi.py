@@ -30,7 +30,7 @@ def test_headers_declare_reference_api():
     # 19 context + 8 buffer + 3 framebuffer + 5 texture + 24 vec + 10 mat functions
     reference_api = [n for n in names if not n.startswith("srpHip") and "Neural" not in n and "Batch" not in n and n not in (
         "srpFramebufferSetDeferred", "srpFramebufferDownload", "srpFramebufferUpload", "srpFramebufferFillDepth",
-        "srpFramebufferFillStencil", "srpNewTextureFromMemory", "srpTextureGetFilteredColorHost", "srpMakeVec2",
+        "srpFramebufferFillStencil", "srpFramebufferSaveImage", "srpNewTextureFromMemory", "srpTextureGetFilteredColorHost", "srpMakeVec2",
         "srpMakeVec3", "srpMakeVec4", "srpMakeVec4From3", "srpMat4Diag_")]
     assert len(reference_api) == 69, reference_api
 

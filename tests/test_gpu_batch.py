@@ -124,3 +124,87 @@ def test_autotuner_picks_a_candidate_and_keeps_results(gpu_batch):
         assert frames_equal(gb.read_all(), base) == []
     finally:
         gb.finish()
+
+
+def _callback(lib, got):
+    import ctypes
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p)
+
+    class Callback(ctypes.Structure):
+        _fields_ = [("func", CB), ("userParameter", ctypes.c_void_p)]
+
+    cb = CB(lambda kind, sev, fn, text, user: got.append((fn.decode(), text.decode())))
+    lib.srpSetMessageCallback.argtypes = [Callback]
+    lib.srpSetMessageCallback(Callback(cb, None))
+    return lambda: lib.srpSetMessageCallback(Callback(CB(), None)), cb
+
+
+@pytest.mark.parametrize("captured", [False, True])
+def test_batch_follows_new_buffer_data(gpu_batch, captured):
+    """New vertex data for two scenes (their device buffers move) after the
+    batch was made, and after its graph was captured: the next frames are
+    prepared again and equal drawing the scenes one at a time, with no
+    message; the graph is replayed again afterwards."""
+    gb = gpu_batch
+    gb.setup(12, 0.05, 21, batched=True)
+    got = []
+    restore, _keep = _callback(gb.lib, got)
+    try:
+        gb.frame(True)
+        if captured:
+            assert gb.capture() == 0
+            gb.frame(True)
+        gb.regenerate(3, 0x1234)
+        gb.regenerate(7, 0x5678)
+        for r in range(3):
+            gb.frame(True)
+            batched = gb.read_all()
+            gb.frame(False)
+            single = gb.read_all()
+            assert frames_equal(batched, single) == [], f"frame {r}"
+        assert got == []
+    finally:
+        restore()
+        gb.finish()
+
+
+def test_batch_with_freed_buffer_draws_nothing(gpu_batch):
+    """A batch whose vertex buffer was freed reports it through the message
+    callback and draws nothing (the framebuffers keep their contents)."""
+    gb = gpu_batch
+    gb.setup(6, 0.05, 23, batched=True)
+    got = []
+    restore, _keep = _callback(gb.lib, got)
+    try:
+        gb.frame(True)
+        before = gb.read_all()
+        gb.drop_scene(2)
+        gb.frame(True)
+        gb.frame(True)
+        assert [fn for fn, _ in got] == ["srpBatchRender", "srpBatchRender"]
+        assert "freed" in got[0][1]
+        assert frames_equal(gb.read_all(), before) == []
+    finally:
+        restore()
+        gb.finish()
+
+
+def test_new_batch_and_capture_draw_nothing(gpu_batch):
+    """Making a batch and capturing its graph leave the framebuffers as they
+    were: only srpBatchRender draws."""
+    gb = gpu_batch
+    gb.setup(6, 0.05, 25, batched=False)
+    try:
+        gb.frame(False)
+        ref = gb.read_all()
+    finally:
+        gb.finish()
+    gb.setup(6, 0.05, 25, batched=True)
+    try:
+        untouched = gb.read_all()
+        assert gb.capture() == 0
+        assert frames_equal(gb.read_all(), untouched) == []
+        gb.frame(True)
+        assert frames_equal(gb.read_all(), ref) == []
+    finally:
+        gb.finish()

@@ -29,13 +29,26 @@ def _reset(mod, which):
     mod.reset_stats(fn)
 
 
+def depth_cull_on(gpu):
+    return gpu.lib.srpHipGetTileDepthCull() != 0
+
+
 def run_both(oracle, gpu, name, **kw):
+    """Both frames and counters.  With tile-level depth culling on (the
+    library default) the GPU's covered count omits fragments of triangles
+    culled for a whole tile, so only shaded and primitive counts are
+    compared then; the variants below turn culling off and compare the
+    covered count too."""
     print(f"[parity] {name} {kw}", flush=True)
     _reset(oracle, "oracle")
     _reset(gpu, "gpu")
     a = oracle.run(name, **kw)
     b = gpu.run(name, **kw)
-    return a, b, _stats(oracle, "oracle"), _stats(gpu, "gpu")
+    sa, sb = list(_stats(oracle, "oracle")), list(_stats(gpu, "gpu"))
+    if depth_cull_on(gpu):
+        assert sb[0] <= sa[0], f"{name}: GPU covered {sb[0]} above the oracle's {sa[0]}"
+        sa[0] = sb[0] = None
+    return a, b, sa, sb
 
 
 def assert_same(a, b, sa, sb, label):
@@ -93,13 +106,16 @@ def test_cfg_terrain_full_grid(oracle, gpu, pose):
     assert_same(a, b, sa, sb, f"cfg/terrain pose {pose}")
 
 
-@pytest.mark.parametrize("batched,runs,box", [(1, 0, 24), (1, 1, 0), (1, 0, 0), (0, 1, 24), (1, 1, 8)])
-def test_raster_variants(oracle, gpu, batched, runs, box):
-    """Every single-draw raster variant gives the oracle's frames."""
+@pytest.mark.parametrize("batched,runs,box,cull", [(1, 0, 24, 1), (1, 1, 0, 1), (1, 0, 0, 1), (0, 1, 24, 1),
+                                                   (1, 1, 8, 1), (1, 1, 24, 0), (1, 1, 0, 0)])
+def test_raster_variants(oracle, gpu, batched, runs, box, cull):
+    """Every single-draw raster variant gives the oracle's frames (and,
+    with depth culling off, its covered counts)."""
     import ctypes
     lib = gpu.lib
     lib.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
     lib.srpHipSetRasterOptions(batched, runs, box)
+    lib.srpHipSetTileDepthCull(cull)
     try:
         cases = [dict(name="cfg/tris", p0=2000, seed=0x5EED0001),
                  dict(name="cfg/mix", width=480, height=270, p0=20000, p1=4000, p2=4000, seed=3),
@@ -108,9 +124,10 @@ def test_raster_variants(oracle, gpu, batched, runs, box):
         cases += [dict(name="cfg/fuzz", seed=s) for s in range(0, 40, 3)]
         for c in cases:
             a, b, sa, sb = run_both(oracle, gpu, **c)
-            assert_same(a, b, sa, sb, f"{c} variant {(batched, runs, box)}")
+            assert_same(a, b, sa, sb, f"{c} variant {(batched, runs, box, cull)}")
     finally:
         lib.srpHipSetRasterOptions(1, 1, 24)
+        lib.srpHipSetTileDepthCull(1)
 
 
 def test_cfg_mesh_small(oracle, gpu, noise_tex):
@@ -136,17 +153,19 @@ def test_fuzz(oracle, gpu, seed):
 
 
 def test_fuzz_sweep_raster_variants(oracle, gpu):
-    """1,000 more randomised scenes, each under one of five rasteriser
+    """1,000 more randomised scenes, each under one of six rasteriser
     configurations in turn (batched with and without pixel runs, mask box
-    24 / 8 / 0, and the simple per-pixel kernel)."""
+    24 / 8 / 0, tile depth culling on and off, and the simple per-pixel
+    kernel)."""
     lib = gpu.lib
     lib.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
-    variants = [(1, 1, 24), (1, 0, 24), (1, 1, 0), (1, 1, 8), (0, 1, 24)]
+    variants = [(1, 1, 24, 1), (1, 0, 24, 1), (1, 1, 0, 1), (1, 1, 8, 1), (0, 1, 24, 1), (1, 1, 0, 0)]
     bad = []
     try:
         for seed in range(200, 1200):
             v = variants[seed % len(variants)]
-            lib.srpHipSetRasterOptions(*v)
+            lib.srpHipSetRasterOptions(*v[:3])
+            lib.srpHipSetTileDepthCull(v[3])
             a = oracle.run("cfg/fuzz", seed=seed)
             b = gpu.run("cfg/fuzz", seed=seed)
             if not (np.array_equal(a.color, b.color) and np.array_equal(a.depth.view(np.uint32), b.depth.view(np.uint32))
@@ -154,6 +173,7 @@ def test_fuzz_sweep_raster_variants(oracle, gpu):
                 bad.append((seed, v))
     finally:
         lib.srpHipSetRasterOptions(1, 1, 24)
+        lib.srpHipSetTileDepthCull(1)
     print(f"[parity] fuzz sweep: 1000 seeds, {len(bad)} differing", flush=True)
     assert not bad, f"differing (seed, options): {bad[:10]}"
 

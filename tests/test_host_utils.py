@@ -93,3 +93,48 @@ def test_obj_teapot(lib):
     assert verts.shape == want.shape == (3498, 8)
     assert np.array_equal(verts, want)
     assert np.array_equal(idx, np.arange(3498, dtype=np.uint32))
+
+
+def test_obj_bad_face_is_skipped_whole(lib, tmp_path):
+    """A face with an out-of-range second or third corner is skipped as a
+    whole (nothing of it pushed), so the faces after it stay aligned."""
+    class Mesh(ctypes.Structure):
+        _fields_ = [("vertices", ctypes.POINTER(ctypes.c_float)), ("vertexCount", ctypes.c_size_t),
+                    ("indices", ctypes.POINTER(ctypes.c_uint32)), ("indexCount", ctypes.c_size_t)]
+    lines = ["v 0 0 0", "v 1 0 0", "v 0 1 0", "v 1 1 0", "vt 0 0", "vt 1 1", "vn 0 0 1",
+             "f 1/1/1 2/2/1 3/1/1",
+             "f 1/1/1 9/2/1 3/1/1",      # second corner out of range
+             "f 2/1/1 3/2/1 7/1/1",      # third corner out of range
+             "f 2/1/1 4/2/1 3/1/1"]
+    path = tmp_path / "bad.obj"
+    path.write_text("\n".join(lines) + "\n")
+    m = Mesh()
+    assert lib.srpObjLoad(str(path).encode(), ctypes.byref(m))
+    verts = np.ctypeslib.as_array(m.vertices, shape=(m.vertexCount, 8)).copy()
+    idx = np.ctypeslib.as_array(m.indices, shape=(m.indexCount,)).copy()
+    lib.srpObjFree(ctypes.byref(m))
+    assert m.indexCount == 6 and idx.tolist() == list(range(6))
+    assert verts[:, :3].tolist() == [[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+
+
+def _png_chunk(kind, data):
+    import struct
+    import zlib
+    return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("w,h", [(1 << 31, 1 << 31), (1 << 25, 1), (1, (1 << 24) + 1), (0x40000000, 4)])
+def test_png_oversized_header_is_refused(lib, tmp_path, w, h):
+    """An IHDR whose sizes would overflow the decoder's buffer arithmetic
+    (the row length times the height wraps in 32 bits for 2^31 x 2^31
+    RGBA8) is refused before any allocation, as stb_image refuses sides
+    above 2^24, with a small valid IDAT behind it."""
+    import struct
+    import zlib
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)
+    png = b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", ihdr) + _png_chunk(b"IDAT", zlib.compress(b"\x00" * 5)) + \
+        _png_chunk(b"IEND", b"")
+    path = tmp_path / "huge.png"
+    path.write_bytes(png)
+    got, err = load_png(lib, str(path))
+    assert got is None and err == "image too large"
