@@ -160,10 +160,67 @@ def gpu_frames(wl, tex, steps, warmup):
     }, frame
 
 
+REF_EXE_DIR = os.path.join(REPO, "oracle", "_ref")
+
+
+def ref_exe():
+    """The reference built as it ships (oracle/Makefile target refbench: -O3
+    -DNDEBUG, one executable) for the highest x86-64 level this host supports
+    (its -march=native stands for the host that builds it), or None."""
+    flags = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        pass
+    levels = []
+    if {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl", "avx2", "fma", "bmi2"} <= flags:
+        levels.append("v4")
+    if {"avx2", "fma", "bmi1", "bmi2", "movbe"} <= flags:
+        levels.append("v3")
+    for lv in levels:
+        p = os.path.join(REF_EXE_DIR, f"ref_bench_{lv}")
+        if os.path.exists(p):
+            return p
+    return None
+
+
+def run_ref_exe(exe, wl, tex, budget_s, min_frames, max_frames):
+    """Frames of the as-shipped reference on one host core (a child process
+    pinned there): per-frame seconds and the last frame."""
+    import subprocess
+    k = wl["kind"]
+    keys = {"mesh": ("rings", "segs"), "tris": ("tris", "seed"), "mix": ("tris", "lines", "points", "seed"),
+            "terrain": ("gx", "gz", "pose")}[k]
+    args = [wl["w"], wl["h"]] + [wl[key] for key in keys] + ([tex or ""] if k == "mesh" else [])
+    dump = os.path.join(tempfile.mkdtemp(prefix="srp_ref_"), "frame.bin")
+    core = min(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    pre = (lambda: os.sched_setaffinity(0, {core})) if core is not None else None
+    r = subprocess.run([exe, k] + [str(a) for a in args] + [str(budget_s), str(min_frames), str(max_frames), dump],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, preexec_fn=pre, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference timing driver failed: {r.stderr}")
+    times = json.loads(r.stdout.strip().splitlines()[-1])["frame_s"]
+    n = wl["w"] * wl["h"]
+    b = np.fromfile(dump, np.uint8)
+    frame = (b[:4 * n].view(np.uint32).copy(), b[4 * n:8 * n].view(np.float32).copy(), b[8 * n:9 * n].copy())
+    os.remove(dump)
+    os.rmdir(os.path.dirname(dump))
+    return times, frame
+
+
 def reference_frames(wl, tex, budget_s, min_frames, max_frames):
-    """The reference renderer compiled in place (oracle/_ref), or the oracle
-    where it was not built, timed on one host core: frames until budget_s
-    (at least min_frames, at most max_frames) after one warm-up frame."""
+    """The reference renderer timed on one host core: built as it ships
+    (ref_exe), else compiled in place as a shared object (oracle/_ref), else
+    the oracle; frames until budget_s (at least min_frames, at most
+    max_frames) after one warm-up frame."""
+    exe = ref_exe()
+    if exe:
+        times, frame = run_ref_exe(exe, wl, tex, budget_s, min_frames, max_frames)
+        return "reference", times, frame
     path = REF_SO if os.path.exists(REF_SO) else ORACLE_SO
     lib = load(path)
     setup(lib, wl, tex)
@@ -279,10 +336,32 @@ def on_one_core(fn):
 
 
 def reference_baseline(args, wl, tex, budget_s, port):
-    """The reference renderer compiled in place (oracle/_ref, built from its
-    own sources with its own flags) timed on one host core for ~budget_s;
-    its shaded count per frame is the oracle's (the frames are equal, which
-    is checked).  Returns (baseline dict, frame)."""
+    """The reference renderer timed on one host core for ~budget_s: built
+    as it ships (one -O3 executable, ref_exe) where that was built, else
+    compiled in place as a shared object (oracle/_ref); its shaded count per
+    frame is the oracle's (the frames are equal, which is checked).  Returns
+    (baseline dict, frame)."""
+    exe = ref_exe()
+    if exe:
+        times, frame = run_ref_exe(exe, wl, tex, budget_s, 5, 200)
+        median = float(np.median(times))
+        model, ncpu = host_cpu()
+        shaded = port["shaded_per_frame"]
+        return {
+            "value": round(shaded / median / 1e6, 4),
+            "unit": "shaded Mfrags/s",
+            "cores": 1,
+            "kind": "reference",
+            "sample": f"{len(times)} full frames of the same workload by the reference built as it ships "
+                      f"({os.path.basename(exe)}: -O3 -DNDEBUG -march=x86-64-{os.path.basename(exe)[-2:]}) after "
+                      f"one warm-up frame, {sum(times):.1f} s, median {median * 1e3:.1f} ms/frame (value is the "
+                      f"median frame; shaded count per frame from the oracle, whose frame it equals)",
+            "ms_per_frame": round(sum(times) / len(times) * 1e3, 3),
+            "ms_per_frame_median": round(median * 1e3, 3),
+            "port_ms_per_frame": port["ms_per_frame"],
+            "host_cpu": model,
+            "host_cpus_visible": ncpu,
+        }, frame
     lib = load(REF_SO)
     setup(lib, wl, tex)
 
@@ -432,7 +511,11 @@ def neural_leg(args):
             nn.use_fb(0)
             return ps, mx
 
+        # poses 0-255 give the training fragments, 320-351 are the validation
+        # poses the encoding was chosen on (--nn-select), and 256-319 are the
+        # test poses, used for nothing but this report
         test_poses = range(256, 256 + 64)
+        val_poses = range(320, 320 + 32)
         ps0, _ = evaluate(test_poses[:8])
         nn.sync()
         t0 = time.perf_counter()
@@ -444,12 +527,32 @@ def neural_leg(args):
                         "loss_first": float(loss[0]), "loss_last": float(np.mean(loss[-50:])),
                         "mfma_tflops": round(samples * NN_TRAIN_FLOPS / t_train / 1e12, 2),
                         "mfma_frac": round(samples * NN_TRAIN_FLOPS / t_train / 1e12 / MFMA_BF16_PEAK_TFLOPS, 5)}
+        pv, mv = evaluate(val_poses)
         ps, mx = evaluate(test_poses)
+        out["psnr_db_val_mean"] = round(float(np.mean(pv)), 3)
+        out["psnr_db_val_min"] = round(float(np.min(pv)), 3)
         out["psnr_db_mean"] = round(float(np.mean(ps)), 3)
         out["psnr_db_min"] = round(float(np.min(ps)), 3)
         out["max_abs_err"] = float(np.max(mx))
         out["psnr_db_untrained"] = round(float(np.mean(ps0)), 3)
-        out["eval_poses"] = len(test_poses)
+        out["eval_poses"] = {"test": [test_poses.start, test_poses.stop - 1],
+                             "validation": [val_poses.start, val_poses.stop - 1], "training": [0, 255]}
+
+        # the network and the procedural material alone, dense over the same
+        # captured fragments (no rasteriser around them)
+        nd = min(int(got), 1 << 20)
+        ms_net = nn.time_inference(net, ds, nd, 10)
+        feats, _ = nn.ds_read(ds, 0, nd)
+        ms_plain = nn.terrain_dense_ms(feats, nd, 10)
+        ps_net, ps_plain = ms_net * 1e9 / nd, ms_plain * 1e9 / nd
+        ps_ideal = NN_FWD_FLOPS / (MFMA_BF16_PEAK_TFLOPS * 1e12) * 1e12
+        out["dense"] = {"fragments": nd, "network_ps_per_fragment": round(ps_net, 2),
+                        "procedural_ps_per_fragment": round(ps_plain, 2),
+                        "network_speedup": round(ps_plain / ps_net, 3) if ps_net > 0 else None,
+                        "network_mfma_tflops": round(NN_FWD_FLOPS / ps_net, 2) if ps_net > 0 else None,
+                        "network_mfma_frac": round(NN_FWD_FLOPS / ps_net / MFMA_BF16_PEAK_TFLOPS, 5) if ps_net > 0 else None,
+                        "network_ps_at_full_mfma": round(ps_ideal, 2),
+                        "speedup_bound_at_full_mfma": round(ps_plain / ps_ideal, 3)}
 
         def time_frames(capture, fs, netdev, pose=256):
             nn.use_fb(0)
@@ -496,6 +599,61 @@ def neural_leg(args):
         nn.free_ds(ds)
         nn.finish()
     return out
+
+
+NN_SELECT_CANDIDATES = [
+    [0.5] * 8,
+    [0.75, 0.5, 0.5, 0.5, 0.5, 0.5, 1.25, 0.75],
+    [1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 2.0, 1.0],
+    [1.0, 0.5, 0.75, 0.5, 0.5, 0.5, 2.0, 1.0],
+    [1.5, 0.5, 0.75, 0.5, 0.5, 0.5, 3.0, 1.5],
+    [1.0, 0.75, 0.5, 0.25, 0.25, 0.25, 2.0, 1.0],
+]
+
+
+def neural_select(args):
+    """Choose the encoding's per-feature frequency steps on the validation
+    poses 320-351 (never on the test poses 256-319): each candidate network
+    is trained as the neural leg trains (same data, schedule and seed) and
+    scored by its mean PSNR over the validation poses."""
+    from neural_runner import Neural
+    nn = Neural()
+    if nn.terrain_setup(args.width, args.height, args.grid_x, args.grid_z, 0, 0) != 0:
+        raise RuntimeError("terrain setup failed")
+    ds = nn.new_ds(8_000_000)
+    table = []
+    try:
+        got = nn.terrain_collect(ds, 0, 256, args.nn_keep, 0x5EED0005)
+        nn.sync()
+        for steps in NN_SELECT_CANDIDATES:
+            net = nn.new_net(0x5EED0005)
+            try:
+                nn.set_encoding(net, steps)
+                nn.train(net, ds, args.nn_steps, args.nn_batch, args.nn_lr, args.nn_lr * 0.02, 0x5EED0005)
+                dev = nn.net_device(net)
+                ps = []
+                for pose in range(320, 352):
+                    nn.terrain_pose(pose)
+                    nn.use_fb(0)
+                    nn.terrain_shader(0, None, None)
+                    nn.frame()
+                    fa = nn.fb()
+                    nn.use_fb(1)
+                    nn.terrain_shader(0, nn.neural_fs, dev)
+                    nn.frame()
+                    ps.append(nn.compare(fa, nn.fb(), fa)[0])
+                nn.use_fb(0)
+                table.append({"freq_steps": steps, "psnr_db_val_mean": round(float(np.mean(ps)), 3),
+                              "psnr_db_val_min": round(float(np.min(ps)), 3)})
+                print(f"bench.py: encoding {steps}: validation {np.mean(ps):.3f} dB", file=sys.stderr, flush=True)
+            finally:
+                nn.free_net(net)
+    finally:
+        nn.free_ds(ds)
+        nn.finish()
+    best = max(table, key=lambda r: r["psnr_db_val_mean"])
+    return {"train_fragments": int(got), "steps": args.nn_steps, "validation_poses": [320, 351],
+            "candidates": table, "best": best["freq_steps"]}
 
 
 def batch_leg(args):
@@ -608,11 +766,13 @@ def main():
     ap.add_argument("--nn-batch", type=int, default=8192)
     ap.add_argument("--nn-lr", type=float, default=5e-3)
     ap.add_argument("--nn-frames", type=int, default=50)
-    ap.add_argument("--nn-freq", default="1.0,0.5,0.5,0.5,0.5,0.5,2.0,1.0",
+    ap.add_argument("--nn-freq", default="1.0,0.5,0.75,0.5,0.5,0.5,2.0,1.0",
                     help="encoding frequency step: one value, or 8 comma-separated (per feature: position xyz, "
-                         "normal xyz, uv); the default doubles x and quadruples u over the library's 0.5, which "
-                         "the terrain's fBm albedo (period ~1/12 of the x extent) needs")
+                         "normal xyz, uv); the default is the best of NN_SELECT_CANDIDATES on the validation "
+                         "poses 320-351 (--nn-select), never on the test poses 256-319 that the leg reports")
     ap.add_argument("--neural-only", action="store_true", help="run only the neural-shader leg and print its JSON")
+    ap.add_argument("--nn-select", action="store_true",
+                    help="only choose the encoding's frequency steps on the validation poses and print the table")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural-shader leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-frames", type=int, default=20)
@@ -629,6 +789,9 @@ def main():
 
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
+        return
+    if args.nn_select:
+        print(json.dumps({"neural_select": neural_select(args)}))
         return
     if args.neural_only:
         print(json.dumps({"neural": neural_leg(args)}))
@@ -717,7 +880,7 @@ def main():
         # the oracle (a C restatement of the reference) checks the frame and
         # counts covered fragments and box pixels; the compiled reference,
         # where it was built, is the timed baseline
-        have_ref = os.path.exists(REF_SO)
+        have_ref = os.path.exists(REF_SO) or ref_exe() is not None
         port_budget = min(args.cpu_budget, 3.0) if have_ref else args.cpu_budget
         cpu, cframe, cpu_shaded = cpu_baseline(args, wl, tex, port_budget)
         bbox_per_frame = cpu["bbox_pixels_per_frame"]

@@ -67,6 +67,10 @@ void srpNeuralNetGradients(SRPNeuralNet* net, const SRPNeuralDataset* ds, size_t
 /** Network output for dataset samples [0, count): rgb[count][3].
  *  lanes = 0: matrix-core path (64 samples per wave); 1: per-lane path. */
 void srpNeuralNetInfer(const SRPNeuralNet* net, const SRPNeuralDataset* ds, size_t count, float* rgb, int lanes);
+/** Device time of the matrix-core inference over dataset samples [0, count)
+ *  (the network alone, dense: 64 samples per wave), mean ms per pass over
+ *  `reps` passes after one warm-up pass. */
+double srpNeuralNetTimeInference(const SRPNeuralNet* net, const SRPNeuralDataset* ds, size_t count, int reps);
 /** Device address of the network's SrpNnDevice view, for a uniform. */
 const void* srpNeuralNetDevice(const SRPNeuralNet* net);
 /** Frequency step of each of the SRP_NN_IN features' encodings (feature d

@@ -155,15 +155,21 @@ struct Runtime
 	void collect()
 	{
 		HIP_CHECK(hipStreamSynchronize(stream));
+		/* one event may bound several spans (a draw's start); each goes
+		 * back to the pool once */
+		std::vector<hipEvent_t> done;
 		for (const ProfRec& r : profPending)
 		{
 			float ms = 0.0f;
 			HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
 			profMs[r.slot] += ms;
 			profLaunches[r.slot]++;
-			eventPool.push_back(r.a);
-			eventPool.push_back(r.b);
+			done.push_back(r.a);
+			done.push_back(r.b);
 		}
+		std::sort(done.begin(), done.end());
+		done.erase(std::unique(done.begin(), done.end()), done.end());
+		eventPool.insert(eventPool.end(), done.begin(), done.end());
 		profPending.clear();
 	}
 

@@ -463,6 +463,22 @@ T* devAlloc(size_t n)
 	return (T*) p;
 }
 
+/* PSNR, largest difference and pixel count from the blocks' partials. */
+void compareResult(const unsigned long long* hp, unsigned blocks, double* psnr, double* maxAbs, uint64_t* pixels)
+{
+	unsigned long long h[3] = {0, 0, 0};
+	for (unsigned k = 0; k < blocks; k++)
+	{
+		h[0] += hp[3 * k + 0];
+		h[1] = hp[3 * k + 1] > h[1] ? hp[3 * k + 1] : h[1];
+		h[2] += hp[3 * k + 2];
+	}
+	const double mse = h[2] ? (double) h[0] / (3.0 * (double) h[2]) : 0.0;
+	if (psnr) *psnr = mse > 0.0 ? 10.0 * log10(255.0 * 255.0 / mse) : INFINITY;
+	if (maxAbs) *maxAbs = (double) h[1];
+	if (pixels) *pixels = h[2];
+}
+
 }  // namespace
 
 /* -------------------------------------------------------------- host objects */
@@ -933,6 +949,44 @@ void srpNeuralNetInfer(const SRPNeuralNet* Nc, const SRPNeuralDataset* D, size_t
 	NN_CHECK(hipStreamSynchronize(s));
 }
 
+/* Device time of the matrix-core inference over dataset samples [0, count):
+ * mean milliseconds per pass over `reps` passes (HIP events on the library
+ * stream around the kernels only; the first pass is a warm-up). */
+double srpNeuralNetTimeInference(const SRPNeuralNet* Nc, const SRPNeuralDataset* D, size_t count, int reps)
+{
+	SRPNeuralNet* N = const_cast<SRPNeuralNet*>(Nc);
+	if (!N || !D || count == 0 || count > D->size || reps < 1)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid network, dataset, count or repetitions");
+		return 0.0;
+	}
+	hipStream_t s = libStream();
+	if (N->rgbCap < count)
+	{
+		NN_CHECK(hipStreamSynchronize(s));
+		if (N->rgb) NN_CHECK(hipFree(N->rgb));
+		N->rgb = devAlloc<float>(count * 3);
+		N->rgbCap = count;
+	}
+	const unsigned blocks = (unsigned) ((count + 63) / 64);
+	hipEvent_t a, b;
+	NN_CHECK(hipEventCreate(&a));
+	NN_CHECK(hipEventCreate(&b));
+	hipLaunchKernelGGL(k_infer<false>, dim3(blocks), dim3(64), 0, s, (const SrpNnDevice*) N->view,
+	                   (const float*) D->feat, (uint64_t) count, N->rgb);
+	NN_CHECK(hipEventRecord(a, s));
+	for (int r = 0; r < reps; r++)
+		hipLaunchKernelGGL(k_infer<false>, dim3(blocks), dim3(64), 0, s, (const SrpNnDevice*) N->view,
+		                   (const float*) D->feat, (uint64_t) count, N->rgb);
+	NN_CHECK(hipEventRecord(b, s));
+	NN_CHECK(hipEventSynchronize(b));
+	float ms = 0.0f;
+	NN_CHECK(hipEventElapsedTime(&ms, a, b));
+	NN_CHECK(hipEventDestroy(a));
+	NN_CHECK(hipEventDestroy(b));
+	return (double) ms / reps;
+}
+
 /* ------------------------------------------------------------- evaluation */
 
 void srpNeuralCompare(const SRPFramebuffer* a, const SRPFramebuffer* b, const SRPFramebuffer* mask, double* psnr,
@@ -958,15 +1012,5 @@ void srpNeuralCompare(const SRPFramebuffer* a, const SRPFramebuffer* b, const SR
 	NN_CHECK(hipMemcpyAsync(hp.data(), part, 3 * sizeof(unsigned long long) * blocks, hipMemcpyDeviceToHost, s));
 	NN_CHECK(hipStreamSynchronize(s));
 	NN_CHECK(hipFree(part));
-	unsigned long long h[3] = {0, 0, 0};
-	for (unsigned k = 0; k < blocks; k++)
-	{
-		h[0] += hp[3 * k + 0];
-		h[1] = hp[3 * k + 1] > h[1] ? hp[3 * k + 1] : h[1];
-		h[2] += hp[3 * k + 2];
-	}
-	const double mse = h[2] ? (double) h[0] / (3.0 * (double) h[2]) : 0.0;
-	if (psnr) *psnr = mse > 0.0 ? 10.0 * log10(255.0 * 255.0 / mse) : INFINITY;
-	if (maxAbs) *maxAbs = (double) h[1];
-	if (pixels) *pixels = h[2];
+	compareResult(hp.data(), blocks, psnr, maxAbs, pixels);
 }

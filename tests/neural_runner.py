@@ -44,6 +44,9 @@ class Neural:
         self._grad = _sig(L, "srpNeuralNetGradients", None, [_vp, _vp, _c.c_size_t, _f32p, _vp])
         self._infer = _sig(L, "srpNeuralNetInfer", None, [_vp, _vp, _c.c_size_t, _f32p, _c.c_int])
         self.net_device = _sig(L, "srpNeuralNetDevice", _vp, [_vp])
+        self.time_inference = _sig(L, "srpNeuralNetTimeInference", _c.c_double, [_vp, _vp, _c.c_size_t, _c.c_int])
+        self._dense = _sig(S, "srpBenchTerrainDenseMs", _c.c_double, [_f32p, _c.c_size_t, _f32p, _c.c_int])
+        self._bounds = _sig(S, "srpBenchTerrainBounds", None, [_f32p])
         self._set_enc = _sig(L, "srpNeuralNetSetEncoding", None, [_vp, _f32p])
         self._get_enc = _sig(L, "srpNeuralNetGetEncoding", None, [_vp, _f32p])
         self._compare = _sig(L, "srpNeuralCompare", None, [_vp, _vp, _vp, _vp, _vp, _vp])
@@ -63,6 +66,12 @@ class Neural:
         self._read = _sig(S, "srpBenchRead", None, [_vp, _vp, _vp])
         self.finish = _sig(S, "srpBenchFinish", None, [])
         self.neural_fs = _c.cast(S.sc_fsTerrainNeural, _vp).value
+
+    def terrain_dense_ms(self, features, count, reps):
+        """Device ms per dense pass of the procedural material over features."""
+        b = np.zeros(6, np.float32)
+        self._bounds(b)
+        return self._dense(np.ascontiguousarray(features[:count], np.float32), count, b, reps)
 
     # -- data
     def ds_read(self, ds, first, count):
