@@ -87,5 +87,15 @@ void srpNeuralNetGetEncoding(const SRPNeuralNet* net, float* step);
  *  (0..255), and the pixel count. */
 void srpNeuralCompare(const SRPFramebuffer* a, const SRPFramebuffer* b, const SRPFramebuffer* mask,
                       double* psnr, double* maxAbs, uint64_t* pixels);
+/** srpNeuralCompare without allocating anything (no hipMalloc or hipFree,
+ *  no allocating library call): for callers that cannot allocate, such as
+ *  tight evaluation loops and harnesses that forbid device allocation.  It
+ *  has no counterpart in the reference.  Its partial sums live in a static
+ *  24 KiB device array per GPU (kept until the process ends) and a static
+ *  host array; calls that overlap on one GPU share them and race.  Any
+ *  framebuffer size fits (the grid strides; at most 1,024 blocks of
+ *  partials).  Same results as srpNeuralCompare. */
+void srpNeuralCompareNoAlloc(const SRPFramebuffer* a, const SRPFramebuffer* b, const SRPFramebuffer* mask,
+                             double* psnr, double* maxAbs, uint64_t* pixels);
 
 SRP_EXTERN_C_END

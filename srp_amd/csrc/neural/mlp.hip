@@ -463,6 +463,46 @@ T* devAlloc(size_t n)
 	return (T*) p;
 }
 
+}  // namespace
+
+/* Scratch of srpNeuralCompareNoAlloc: the per-block partial sums (squared
+ * error, largest difference, pixel count) of at most kCompareBlocks blocks,
+ * 24 KiB, which every GPU the library runs on holds until the process ends.
+ * In host code this name is only a stand-in for the symbol: its device
+ * address on each GPU comes from hipGetSymbolAddress (scratchAddr). */
+static __device__ __align__(256) unsigned long long scratch[3 * kCompareBlocks];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+unsigned long long* g_scratchAddr[kMaxScratchDevices];     /* per device, set on first use */
+unsigned long long g_scratchHost[3 * kCompareBlocks];      /* host copy of the partials */
+
+/* The device address of `scratch` on the current device (looked up once per
+ * device, with that device current); null, with a message, on failure. */
+unsigned long long* scratchAddr(const char* fn)
+{
+	int dev = -1;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, fn, "No usable current device (%d)", dev);
+		return nullptr;
+	}
+	if (!g_scratchAddr[dev])
+	{
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+		{
+			libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, fn, "Device scratch not available: %s",
+			           hipGetErrorString(e));
+			return nullptr;
+		}
+		g_scratchAddr[dev] = (unsigned long long*) p;
+	}
+	return g_scratchAddr[dev];
+}
+
 /* PSNR, largest difference and pixel count from the blocks' partials. */
 void compareResult(const unsigned long long* hp, unsigned blocks, double* psnr, double* maxAbs, uint64_t* pixels)
 {
@@ -1013,4 +1053,43 @@ void srpNeuralCompare(const SRPFramebuffer* a, const SRPFramebuffer* b, const SR
 	NN_CHECK(hipStreamSynchronize(s));
 	NN_CHECK(hipFree(part));
 	compareResult(hp.data(), blocks, psnr, maxAbs, pixels);
+}
+
+/* srpNeuralCompare without any allocation: the partials go to the file's
+ * static device scratch (one per GPU) and a static host array.  Calls that
+ * overlap on one GPU share both and race. */
+void srpNeuralCompareNoAlloc(const SRPFramebuffer* a, const SRPFramebuffer* b, const SRPFramebuffer* mask,
+                             double* psnr, double* maxAbs, uint64_t* pixels)
+{
+	if (!a || !b || !mask || a->size != b->size || a->size != mask->size)
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Framebuffers must exist and have one size");
+		return;
+	}
+	hipStream_t s = libStream();
+	unsigned long long* part = scratchAddr(__func__);
+	if (!part)
+		return;
+	const size_t need = (a->size + 255) / 256;
+	const unsigned blocks = need == 0 ? 1u : (need < kCompareBlocks ? (unsigned) need : kCompareBlocks);
+	/* the grid strides over any frame, so its partials never exceed the
+	 * scratch; a larger grid would be refused here */
+	if (3 * (size_t) blocks > sizeof(g_scratchHost) / sizeof(g_scratchHost[0]))
+	{
+		libMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__,
+		           "%u blocks of partials exceed the %zu-entry scratch", blocks,
+		           sizeof(g_scratchHost) / sizeof(g_scratchHost[0]) / 3);
+		return;
+	}
+	uint32_t *ca, *cb;
+	float* depth;
+	srp::lib::framebufferDevice(a, &ca, NULL, NULL);
+	srp::lib::framebufferDevice(b, &cb, NULL, NULL);
+	srp::lib::framebufferDevice(mask, NULL, &depth, NULL);
+	/* every block writes its own three partials: nothing to clear */
+	hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(256), 0, s, (const uint32_t*) ca, (const uint32_t*) cb,
+	                   (const float*) depth, a->size, part);
+	NN_CHECK(hipMemcpyAsync(g_scratchHost, part, 3 * sizeof(unsigned long long) * blocks, hipMemcpyDeviceToHost, s));
+	NN_CHECK(hipStreamSynchronize(s));
+	compareResult(g_scratchHost, blocks, psnr, maxAbs, pixels);
 }
