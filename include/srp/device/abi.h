@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 10u
+#define SRP_ABI_VERSION 11u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -203,10 +203,11 @@ typedef struct SrpFragLaunch
 	const uint32_t* order;       /* fragment index for each sorted key */
 	const SrpLine* lines;        /* kind 0 */
 	const SrpPoint* points;      /* kind 1 */
-	uint32_t count;
+	uint32_t count;              /* fragments, or a capacity when countDev is set */
 	uint32_t kind;
 	uint32_t lateZ;
 	uint32_t words;
+	const uint32_t* countDev;    /* if set: the fragment count is min(count, *countDev) */
 } SrpFragLaunch;
 
 typedef void (*SrpVsLaunchFn)(const SrpVsLaunch*, void* stream);

@@ -1125,9 +1125,12 @@ __global__ __launch_bounds__(256) void srp_frag_kernel(SrpFragLaunch L)
 	PixelState ps;
 	ps.color = 0; ps.depth = 0.0f; ps.stencil = 0; ps.covered = 0; ps.shaded = 0;
 	const uint32_t npix = P->width * P->height;
+	uint32_t count = L.count;
+	if (L.countDev && *L.countDev < count)
+		count = *L.countDev;
 	uint32_t key = 0xFFFFFFFFu;
 	bool head = false;
-	if (i < L.count)
+	if (i < count)
 	{
 		key = L.keys[i];
 		head = key < npix && (i == 0 || L.keys[i - 1] != key);
@@ -1138,7 +1141,7 @@ __global__ __launch_bounds__(256) void srp_frag_kernel(SrpFragLaunch L)
 		ps.depth = P->depth[key];
 		ps.stencil = P->stencil[key];
 		const uint32_t pv = P->st.provokingLast ? 1u : 0u;
-		for (uint32_t j = i; j < L.count && L.keys[j] == key; j++)
+		for (uint32_t j = i; j < count && L.keys[j] == key; j++)
 		{
 			const SrpFrag f = L.frags[L.order[j]];
 			if (L.kind == 0)

@@ -606,6 +606,8 @@ __device__ __forceinline__ void fanTriangle(const AsmArgs& A, const AsmDraw& W, 
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 {
+	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+		return;
 	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
 	const AsmDraw& W = A.draws[d];
 	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;
@@ -989,6 +991,8 @@ __device__ __forceinline__ void clipGroup(const AsmArgs& A, uint32_t g, bool has
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_clipped(AsmArgs A)
 {
+	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+		return;
 	const uint32_t n = *A.clipCount;
 	constexpr uint32_t perBlock = 128u / CLIP_GROUP;
 	const uint32_t gl = threadIdx.x & (CLIP_GROUP - 1u);
@@ -1012,6 +1016,8 @@ __device__ __forceinline__ void lineStream(uint32_t prim, uint64_t base, uint64_
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 {
+	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+		return;
 	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
 	const AsmDraw& W = A.draws[d];
 	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;
@@ -1114,6 +1120,8 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 {
+	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+		return;
 	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
 	const AsmDraw& W = A.draws[d];
 	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;

@@ -82,6 +82,9 @@ struct AsmArgs
 	/* capacities of the triangle outputs (records, tile pairs, generated
 	 * vertices); what lies beyond them is dropped and flagged */
 	uint32_t recCap, itemCap, genCap;
+	/* emitting pass: when set and nonzero on the device, the pass does
+	 * nothing (the counted totals did not fit the capacities; see k_fit) */
+	const uint32_t* skip;
 };
 
 /* kind: 0 triangles, 1 lines, 2 points.  clipped: how many triangles the
@@ -133,6 +136,20 @@ struct SortScratch
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& scratch,
                uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr);
 void radixSortReserve(SortScratch& scratch, size_t n);
+
+/* Capacities an emitting pass was sized for before its counts were known. */
+struct FitCaps
+{
+	unsigned long long records, generated, items, spans;
+};
+/* Whether a counting pass's exact totals (totals64: records, generated
+ * vertices, items; *spanCount: spans) fit `caps`.  One thread writes
+ * *skip = 0 if they do, 1 if they do not, 2 if there is nothing to emit,
+ * and *items = the item count if they fit, else 0: the emitting pass, the
+ * sort and the tile bounds read these on the device, so a draw that does
+ * not fit leaves every buffer unchanged. */
+void launchFit(const unsigned long long* totals64, const uint32_t* spanCount, FitCaps caps, uint32_t* skip,
+               uint32_t* items, hipStream_t s);
 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 /* *dst = *src by a one-thread kernel (dst may be host-mapped memory) */

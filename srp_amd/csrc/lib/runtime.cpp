@@ -1,10 +1,12 @@
 /* Host side of libsrp_amd: the srp C API (include/srp) over the HIP
  * pipeline.  Entry points keep the reference's names, argument meaning and
  * error behaviour (message callback, early return; kitrofimov/srp
- * core/*.c, pipeline/draw.c); the work of a draw runs on the GPU:
+ * core/ and pipeline/draw.c); the work of a draw runs on the GPU:
  *
  *   vertex shader (shader module)  ->  assembly/clip/setup COUNT pass
- *   -> scans -> one host sync for the totals -> EMIT pass
+ *   -> scans -> EMIT pass, sized from the same draw's previous counts and
+ *      checked against its counts on the device (the host reads them at its
+ *      next call; a new draw, or one that outgrew its buffers, waits)
  *   -> filled triangles: stable radix sort of (tile, primitive) pairs,
  *      per-tile rasteriser (shader module)
  *   -> lines/points: stable radix sort of fragments by pixel, per-pixel
@@ -57,9 +59,13 @@ SRPContext* g_ctx = nullptr;
 
 SRPContext* ctx() { return g_ctx ? g_ctx : &srpContext; }
 
-/* utils/message_callback.c:18-35 */
+void resolvePending();
+
+/* utils/message_callback.c:18-35.  A draw still waiting for its counts
+ * reports first, so messages keep the order of the calls that caused them. */
 void message(SRPMessageType type, SRPMessageSeverity sev, const char* fn, const char* fmt, ...)
 {
+	resolvePending();
 	SRPContext* c = ctx();
 	if (c->messageCallback.func == nullptr)
 		return;
@@ -108,6 +114,32 @@ constexpr size_t kMaxInlineUniform = 16384;
 constexpr size_t kStatWords = (size_t) SRP_STAT_SLOTS * SRP_STAT_STRIDE;
 constexpr size_t kParamsBlock = kParamsHead + kMaxInlineUniform;
 
+/* A single draw as the counts of its emitting half depend on it: the
+ * buffers and range it reads and the framebuffer's size. */
+struct DrawKey
+{
+	const void* ib;
+	const void* vb;
+	size_t start, count;
+	uint32_t prim, width, height;
+	bool operator==(const DrawKey& o) const
+	{
+		return ib == o.ib && vb == o.vb && start == o.start && count == o.count && prim == o.prim &&
+		       width == o.width && height == o.height;
+	}
+};
+struct DrawKeyHash
+{
+	size_t operator()(const DrawKey& k) const
+	{
+		size_t h = std::hash<const void*>()(k.ib) * 31u ^ std::hash<const void*>()(k.vb);
+		h = h * 1000003u ^ k.start;
+		h = h * 1000003u ^ k.count;
+		h = h * 1000003u ^ ((size_t) k.prim << 40 ^ (size_t) k.width << 20 ^ k.height);
+		return h;
+	}
+};
+
 struct Runtime
 {
 	hipStream_t stream = nullptr;
@@ -123,7 +155,34 @@ struct Runtime
 	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
 	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
 	uint32_t depthCull = 1; /* tile-level depth culling in runs */
-	uint32_t lastClipped = 0;   /* triangles the previous draw clipped (grid hint) */
+	uint32_t lastClipped = 0;   /* triangles the previous draw clipped (grid hint of a new draw) */
+
+	/* A single draw whose emitting half was enqueued before its counts
+	 * reached the host (see PendingDraw). */
+	struct Pending
+	{
+		bool active = false;
+		int kind = 0, backend = 0;
+		uint32_t nIn = 0, nTiles = 0, cacheSize = 0, varyingsSize = 0, lateZ = 0, words = 0;
+		uint32_t clipHint = 0;
+		FitCaps last = {0, 0, 0, 0};   /* the same draw's counts at its last call */
+		int fragBits = 0;
+		bool emitted = false;      /* the emitting half is enqueued (else it waits for the counts) */
+		DrawKey key;
+		AsmArgs A;                 /* counting-pass arguments: offsets, lists, counters */
+		SrpRasterLaunch rl;        /* fields known before the counts */
+		const SrpFsEntry* fse = nullptr;
+		hipEvent_t evDraw = nullptr;
+	} pending;
+	hipEvent_t countsCopied = nullptr;  /* the pending draw's counters are on the host */
+	/* each draw's counts at its last call, which size its next emitting
+	 * half, and how many triangles it clipped (the clip kernels' grids) */
+	struct DrawCounts
+	{
+		FitCaps counts;
+		uint32_t clipped;
+	};
+	std::unordered_map<DrawKey, DrawCounts, DrawKeyHash> drawCounts;
 
 	DevBuf uniform, clip, varyings, genVaryings, counts;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount, spans;
@@ -183,6 +242,7 @@ struct Runtime
 		HIP_CHECK(hipMalloc((void**) &dStats, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipMemset(dStats, 0, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
+		HIP_CHECK(hipEventCreateWithFlags(&countsCopied, hipEventDisableTiming));
 		const char* t = getenv("SRP_TILE");
 		if (t && (atoi(t) == 8 || atoi(t) == 16)) tile = atoi(t);
 		const char* b = getenv("SRP_RASTER");
@@ -191,9 +251,11 @@ struct Runtime
 	}
 };
 
+Runtime* g_rt = nullptr;   /* set once the runtime exists */
+
 Runtime& rt()
 {
-	static Runtime* r = new Runtime();
+	static Runtime* r = (g_rt = new Runtime());
 	return *r;
 }
 
@@ -317,7 +379,11 @@ void texPush(TexImpl* T)
 namespace srp {
 namespace lib {
 
-hipStream_t libStream() { return rt().stream; }
+hipStream_t libStream()
+{
+	resolvePending();
+	return rt().stream;
+}
 
 void libMessage(SRPMessageType type, SRPMessageSeverity sev, const char* fn, const char* fmt, ...)
 {
@@ -331,6 +397,7 @@ void libMessage(SRPMessageType type, SRPMessageSeverity sev, const char* fn, con
 
 void framebufferDevice(const SRPFramebuffer* fb, uint32_t** color, float** depth, uint8_t** stencil)
 {
+	resolvePending();
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 		fbFlushFills(F);
@@ -420,6 +487,7 @@ extern "C" {
 /* core/context.c:15-51 */
 void srpNewContext(SRPContext* c)
 {
+	resolvePending();
 	g_ctx = c;
 	c->messageCallback.func = nullptr;
 	c->messageCallback.userParameter = nullptr;
@@ -443,7 +511,11 @@ void srpNewContext(SRPContext* c)
 }
 
 /* core/context.c:53-180 */
-void srpSetMessageCallback(SRPMessageCallback cb) { ctx()->messageCallback = cb; }
+void srpSetMessageCallback(SRPMessageCallback cb)
+{
+	resolvePending();   /* a pending draw reports to the callback of its call */
+	ctx()->messageCallback = cb;
+}
 void srpProvokingVertexMode(SRPProvokingVertexMode m) { ctx()->provokingVertexMode = m; }
 void srpRasterCullFace(SRPFace f) { ctx()->raster.cullFace = f; }
 void srpRasterFrontFace(SRPWinding w) { ctx()->raster.frontFace = w; }
@@ -512,6 +584,7 @@ SRPVertexBuffer* srpNewVertexBuffer(void)
 
 void srpFreeVertexBuffer(SRPVertexBuffer* b)
 {
+	resolvePending();
 	if (!b) return;
 	liveObjects().erase(b);
 	if (b->dData)
@@ -525,6 +598,7 @@ void srpFreeVertexBuffer(SRPVertexBuffer* b)
 /* core/buffer.c:30-45 (the copy lands in device memory) */
 void srpVertexBufferCopyData(SRPVertexBuffer* b, size_t stride, size_t nBytes, const void* data)
 {
+	resolvePending();
 	Runtime& R = rt();
 	if (nBytes > b->nBytesAllocated)
 	{
@@ -553,6 +627,7 @@ SRPIndexBuffer* srpNewIndexBuffer(void)
 
 void srpFreeIndexBuffer(SRPIndexBuffer* b)
 {
+	resolvePending();
 	if (!b) return;
 	liveObjects().erase(b);
 	if (b->dData)
@@ -583,6 +658,7 @@ static size_t sizeofIndexType(SRPType t)
 /* core/buffer.c:77-94 */
 void srpIndexBufferCopyData(SRPIndexBuffer* b, SRPType type, size_t nBytes, const void* data)
 {
+	resolvePending();
 	Runtime& R = rt();
 	if (nBytes > b->nBytesAllocated)
 	{
@@ -645,6 +721,7 @@ SRPFramebuffer* srpNewFramebuffer(size_t w, size_t h)
 
 void srpFreeFramebuffer(SRPFramebuffer* fb)
 {
+	resolvePending();
 	if (!fb) return;
 	liveObjects().erase(fb);
 	FbImpl* F = fbImpl(fb);
@@ -661,6 +738,7 @@ void srpFreeFramebuffer(SRPFramebuffer* fb)
 /* core/framebuffer.c:57-62: colour 0, depth -1, stencil untouched */
 void srpFramebufferClear(const SRPFramebuffer* fb)
 {
+	resolvePending();
 	FbImpl* F = fbImpl(fb);
 	if (F->deferred)
 	{
@@ -677,6 +755,7 @@ void srpFramebufferClear(const SRPFramebuffer* fb)
 
 void srpFramebufferSetDeferred(const SRPFramebuffer* fb, bool deferred)
 {
+	resolvePending();
 	if (!fb)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
@@ -692,6 +771,7 @@ void srpFramebufferSetDeferred(const SRPFramebuffer* fb, bool deferred)
 
 void srpFramebufferDownload(const SRPFramebuffer* fb)
 {
+	resolvePending();
 	if (!fb)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
@@ -705,6 +785,7 @@ void srpFramebufferDownload(const SRPFramebuffer* fb)
  * copy is downloaded first. */
 int srpFramebufferSaveImage(const SRPFramebuffer* fb, const char* path)
 {
+	resolvePending();
 	if (!fb || !path)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer or path");
@@ -724,6 +805,7 @@ int srpFramebufferSaveImage(const SRPFramebuffer* fb, const char* path)
 
 void srpFramebufferUpload(const SRPFramebuffer* fb)
 {
+	resolvePending();
 	if (!fb)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
@@ -735,6 +817,7 @@ void srpFramebufferUpload(const SRPFramebuffer* fb)
 
 void srpFramebufferFillDepth(const SRPFramebuffer* fb, float v)
 {
+	resolvePending();
 	if (!fb)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
@@ -752,6 +835,7 @@ void srpFramebufferFillDepth(const SRPFramebuffer* fb, float v)
 
 void srpFramebufferFillStencil(const SRPFramebuffer* fb, uint8_t v)
 {
+	resolvePending();
 	if (!fb)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Invalid framebuffer");
@@ -797,6 +881,7 @@ SRPTexture* srpNewTextureFromMemory(const uint8_t* rgb, int w, int h, SRPTexture
 
 void srpFreeTexture(SRPTexture* t)
 {
+	resolvePending();
 	TexImpl* T = texImpl(t);
 	if (!T) return;
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
@@ -847,6 +932,7 @@ int srpTextureGet(SRPTexture* t, SRPTextureParameter p)
 
 void srpTextureSet(SRPTexture* t, SRPTextureParameter p, int v)
 {
+	resolvePending();
 	TexImpl* T = texImpl(t);
 	if (!T || (p != SRP_TEXTURE_WRAPPING_MODE_X && p != SRP_TEXTURE_WRAPPING_MODE_Y))
 	{
@@ -895,6 +981,7 @@ void srpHipShaderCalledOnHost(const char* name)
 
 void srpHipGetStats(uint64_t out[4])
 {
+	resolvePending();
 	Runtime& R = rt();
 	std::vector<unsigned long long> d(kStatWords);
 	HIP_CHECK(hipStreamSynchronize(R.stream));
@@ -911,6 +998,7 @@ void srpHipGetStats(uint64_t out[4])
 
 void srpHipResetStats(void)
 {
+	resolvePending();
 	Runtime& R = rt();
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 	HIP_CHECK(hipMemset(R.dStats, 0, kStatWords * sizeof(unsigned long long)));
@@ -918,11 +1006,20 @@ void srpHipResetStats(void)
 	R.hostDraws = 0;
 }
 
-void* srpHipGetStream(void) { return (void*) rt().stream; }
-void srpHipSynchronize(void) { HIP_CHECK(hipStreamSynchronize(rt().stream)); }
+void* srpHipGetStream(void)
+{
+	resolvePending();   /* work the caller adds follows the pending draw's, repeated or not */
+	return (void*) rt().stream;
+}
+void srpHipSynchronize(void)
+{
+	resolvePending();
+	HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
 
 void srpHipProfileEnable(bool enable)
 {
+	resolvePending();
 	Runtime& R = rt();
 	R.collect();
 	R.profile = enable;
@@ -931,6 +1028,7 @@ void srpHipProfileEnable(bool enable)
 
 void srpHipProfileRead(double ms[4], uint64_t launches[4])
 {
+	resolvePending();
 	Runtime& R = rt();
 	R.collect();
 	for (int i = 0; i < 4; i++) { ms[i] = R.profMs[i]; launches[i] = R.profLaunches[i]; }
@@ -938,12 +1036,14 @@ void srpHipProfileRead(double ms[4], uint64_t launches[4])
 
 void srpHipSetTileSize(int t)
 {
+	resolvePending();
 	if (t == 8 || t == 16) rt().tile = t;
 }
 int srpHipGetTileSize(void) { return rt().tile; }
 
 void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox)
 {
+	resolvePending();
 	Runtime& R = rt();
 	if (batched >= 0) R.batched = batched ? 1 : 0;
 	if (pixelRuns >= 0) R.bigPixel = pixelRuns ? 1u : 0u;
@@ -957,6 +1057,7 @@ void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox)
 void srp_lib_scan_exact(const uint32_t* counts, size_t n, int nArrays, uint32_t* offsets, uint32_t* totals32,
                         uint64_t* totals64)
 {
+	resolvePending();
 	Runtime& R = rt();
 	uint32_t* d = nullptr;
 	uint32_t* t32 = nullptr;
@@ -979,7 +1080,11 @@ void srp_lib_scan_exact(const uint32_t* counts, size_t n, int nArrays, uint32_t*
 	if (sc.buf64) HIP_CHECK(hipFree(sc.buf64));
 }
 
-void srpHipSetTileDepthCull(int enable) { rt().depthCull = enable ? 1u : 0u; }
+void srpHipSetTileDepthCull(int enable)
+{
+	resolvePending();
+	rt().depthCull = enable ? 1u : 0u;
+}
 int srpHipGetTileDepthCull(void) { return (int) rt().depthCull; }
 
 }  // extern "C"
@@ -1081,9 +1186,166 @@ bool snapshotState(const SRPContext* C, const SRPShaderProgram* sp, int kind, Sr
 	return true;
 }
 
+/* Buffers of a single draw's emitting half for `want` (the counts, or an
+ * estimate of them), the fit check of the counts against those buffers,
+ * and the half itself: emitting pass, span expansion, sort, tile ranges,
+ * rasteriser or fragment lists.  Speculative (the counts are not known
+ * yet): the counts are also copied to the host, for resolvePending. */
+void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
+{
+	Runtime::Pending& D = R.pending;
+	AsmArgs A = D.A;
+	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
+	const unsigned long long* totals64 = reinterpret_cast<const unsigned long long*>(totals + 8);
+	FitCaps caps = {kMaxDrawRecords, kMaxDrawVertexSlots - D.cacheSize, kMaxDrawItems, 0xFFFFFFFFull};
+	auto take = [](DevBuf& b, unsigned long long n, size_t elem, size_t pad, unsigned long long& cap) {
+		uint8_t* p = b.get<uint8_t>((size_t) n * elem + pad);
+		cap = std::min<unsigned long long>(cap, (b.cap - pad) / elem);
+		return p;
+	};
+	/* clip-generated vertices: the device parameters hold the buffer's
+	 * address, so a new buffer is sent there */
+	const size_t vs = D.varyingsSize;
+	uint8_t* genBuf = R.genVaryings.get<uint8_t>((size_t) want.generated * vs + 512);
+	if (vs)
+		caps.generated = std::min<unsigned long long>(caps.generated, (R.genVaryings.cap - 512) / vs);
+	if (genBuf != R.hParams->genVaryings)
+	{
+		HIP_CHECK(hipEventSynchronize(R.paramsCopied));
+		R.hParams->genVaryings = genBuf;
+		HIP_CHECK(hipMemcpyAsync(&R.dParams->genVaryings, &R.hParams->genVaryings, sizeof(genBuf),
+		                         hipMemcpyHostToDevice, R.stream));
+		HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
+	}
+	A.genVaryings = genBuf;
+	uint32_t* tileStart = nullptr;
+	uint32_t* tileEnd = nullptr;
+	if (D.backend == 0)
+	{
+		A.tris = (SrpTri*) take(R.tris, want.records, sizeof(SrpTri), 0, caps.records);
+		A.pairKeys = (uint32_t*) take(R.pairKeys, want.items, 4, 16, caps.items);
+		A.pairVals = (uint32_t*) take(R.pairVals, want.items, 4, 16, caps.items);
+		A.pairRec = (uint32_t*) take(R.pairRec, want.items, 4, 16, caps.items);
+		A.pairMask = (unsigned long long*) take(R.pairMask, want.items, 8, 16, caps.items);
+		A.spans = (SrpSpan*) take(R.spans, want.spans, sizeof(SrpSpan), 64, caps.spans);
+		tileStart = R.tileCount.get<uint32_t>(((size_t) 2 * D.nTiles + 2) * 4);
+		tileEnd = tileStart + D.nTiles;
+	}
+	else
+	{
+		if (D.backend == 1) A.lines = (SrpLine*) take(R.lines, want.records, sizeof(SrpLine), 0, caps.records);
+		else A.points = (SrpPoint*) take(R.points, want.records, sizeof(SrpPoint), 0, caps.records);
+		A.frags = (SrpFrag*) take(R.frags, want.items, sizeof(SrpFrag), 16, caps.items);
+		A.fragKeys = (uint32_t*) take(R.fragKeys, want.items, 4, 16, caps.items);
+		caps.spans = 0;
+	}
+	uint32_t* order = (D.backend != 0) ? (uint32_t*) take(R.order, want.items, 4, 16, caps.items) : nullptr;
+	/* the sort, tile ranges and fragment kernel size their grids by this */
+	caps.items = std::min(caps.items, want.items);
+	A.recCap = (uint32_t) caps.records;
+	A.itemCap = (uint32_t) caps.items;
+	A.genCap = (uint32_t) std::min<unsigned long long>(caps.generated, 0xFFFFFFFFull);
+	A.spanCap = (uint32_t) caps.spans;
+	A.spanCount = totals + 5;   /* emitting pass: append cursor (zeroed by the upload) */
+	uint32_t* skip = totals + 7;
+	A.skip = skip;
+	uint32_t* items = totals + 2;   /* becomes the item count, or 0 when skipped */
+	launchFit(totals64, totals + 6, caps, skip, items, R.stream);
+	if (speculative)
+	{
+		HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
+		HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
+	}
+
+	if (D.backend == 0)
+		fillU32(tileStart, 0u, (size_t) 2 * D.nTiles, R.stream);
+	launchAssemble(A, D.kind, true, R.stream, speculative ? kClipUnknown : R.hTotals[4], D.clipHint);
+	if (D.backend == 0)
+	{
+		/* spans: the grid strides over the device-side count */
+		launchExpandSpans(A, (uint32_t) std::min<unsigned long long>((caps.spans + 3) / 4, 2048ull), R.stream);
+		uint32_t *keys, *vals;
+		radixSort(A.pairKeys, A.pairVals, caps.items, bitsFor(D.nTiles), R.sort, &keys, &vals, R.stream, items);
+		tileBounds(keys, caps.items, tileStart, tileEnd, R.stream, items);
+		SrpRasterLaunch rl = D.rl;
+		rl.tris = A.tris;
+		rl.tileList = vals;
+		rl.tileStart = tileStart;
+		rl.tileEnd = tileEnd;
+		rl.pairRec = A.pairRec;
+		rl.pairMask = A.pairMask;
+		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
+		D.fse->raster(&rl, (void*) R.stream);
+		if (R.profile) R.span(0, e0, R.mark());
+	}
+	else
+	{
+		iota(order, caps.items, R.stream);
+		uint32_t *keys, *vals;
+		radixSort(A.fragKeys, order, caps.items, D.fragBits, R.sort, &keys, &vals, R.stream, items);
+		SrpFragLaunch fl;
+		memset(&fl, 0, sizeof(fl));
+		fl.params = R.dParams;
+		fl.frags = A.frags;
+		fl.keys = keys;
+		fl.order = vals;
+		fl.lines = A.lines;
+		fl.points = A.points;
+		fl.count = (uint32_t) caps.items;
+		fl.countDev = items;
+		fl.kind = (D.backend == 1) ? 0u : 1u;
+		fl.lateZ = D.lateZ;
+		fl.words = D.words;
+		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
+		D.fse->frags(&fl, (void*) R.stream);
+		if (R.profile) R.span(2, e0, R.mark());
+	}
+	if (R.profile)
+		R.span(1, D.evDraw, R.mark());
+}
+
+/* Settle the pending draw, if any: wait for its counts (not for the draw),
+ * report what they show as the synchronous path would have (clip overflow,
+ * a draw over the limits: refused, nothing drawn), and when the buffers it
+ * was given were too small, enqueue its emitting half again with buffers
+ * for its counts (the first attempt was skipped on the device). */
+void resolvePending()
+{
+	if (!g_rt || !g_rt->pending.active)
+		return;
+	Runtime& R = *g_rt;
+	Runtime::Pending& D = R.pending;
+	D.active = false;
+	HIP_CHECK(hipEventSynchronize(R.countsCopied));
+	const uint32_t* h = R.hTotals;
+	const uint64_t* h64 = reinterpret_cast<const uint64_t*>(h + 8);
+	if (h[3] & 1u)
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
+		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
+	const DrawTotals dt = {h64[0], h64[1], h64[2], D.cacheSize};
+	const DrawLimit lim = checkDrawTotals(dt);
+	if (lim != DRAW_FITS)
+	{
+		char text[256];
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "%s", drawLimitText(lim, dt, text, sizeof(text)));
+		return;   /* skipped on the device: nothing drawn */
+	}
+	if (D.kind == 0)
+		R.lastClipped = h[4];
+	R.hostDraws++;
+	R.hostPrims += h64[0];
+	const FitCaps need = {h64[0], h64[1], h64[2], h[6]};
+	if (R.drawCounts.size() >= 4096)
+		R.drawCounts.clear();
+	R.drawCounts[D.key] = {need, h[4]};
+	if (h64[0] != 0u && (!D.emitted || h[7] == 1u))
+		enqueueEmit(R, need, false);
+}
+
 void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFramebuffer* fb,
                 const SRPShaderProgram* sp, SRPPrimitive prim, size_t start, size_t count)
 {
+	resolvePending();
 	SRPContext* C = ctx();
 	/* pipeline/draw.c:63-82, :167-189 */
 	if (count == 0)
@@ -1274,6 +1536,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		AD.firstBlock = 0;
 	}
 	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, paramsBytes, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
 
 	/* Vertex stage (single draw: the multi-draw fields stay zero). */
 	SrpVsLaunch vl;
@@ -1297,133 +1560,72 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	A.error = totals + 3;
 	A.clipCount = totals + 4;
 	A.spanCount = totals + 6;   /* counting pass: how many spans */
-	/* the counting pass's totals are checked (exactly, in 64 bits) before
-	 * the emitting pass, so the capacities can be the totals themselves */
 	A.recCap = A.itemCap = A.genCap = 0xFFFFFFFFu;
 	unsigned long long* totals64 = reinterpret_cast<unsigned long long*>(totals + 8);
-	launchAssemble(A, kind, false, R.stream, kClipUnknown, R.lastClipped);
+	Runtime::Pending& D = R.pending;
+	D.key = {ib, vb, start, count, (uint32_t) prim, H.width, H.height};
+	auto seen = R.drawCounts.find(D.key);
+	D.emitted = seen != R.drawCounts.end();
+	D.clipHint = D.emitted ? seen->second.clipped : R.lastClipped;
+	if (D.emitted)
+		D.last = seen->second.counts;
+	launchAssemble(A, kind, false, R.stream, kClipUnknown, D.clipHint);
 	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream, totals64);
 	if (R.profile) R.span(3, evDraw, R.mark());
-	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
-	HIP_CHECK(hipStreamSynchronize(R.stream));
-	const uint64_t* hTotals64 = reinterpret_cast<const uint64_t*>(R.hTotals + 8);
-	if (R.hTotals[3] & 1u)
-		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
-		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
-	{
-		const DrawTotals dt = {hTotals64[0], hTotals64[1], hTotals64[2], cacheSize};
-		const DrawLimit lim = checkDrawTotals(dt);
-		if (lim != DRAW_FITS)
-		{
-			char text[256];
-			message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "drawBuffer", "%s",
-			        drawLimitText(lim, dt, text, sizeof(text)));
-			return;   /* nothing emitted: the device copy is unchanged */
-		}
-	}
-	const uint32_t nRec = (uint32_t) hTotals64[0], nGen = (uint32_t) hTotals64[1], nItems = (uint32_t) hTotals64[2];
-	const uint32_t nClipped = R.hTotals[4], nSpans = R.hTotals[6];
-	if (kind == 0)
-		R.lastClipped = nClipped;
-	R.hostDraws++;
-	R.hostPrims += nRec;
-	if (nRec == 0)
-		return;   /* nothing drawn: the device copy is unchanged */
 
-	/* EMIT pass. */
-	const int backend = (kind == 0) ? (int) S.polygonMode : (kind == 1 ? 1 : 2);  /* 0 tiles, 1 lines, 2 points */
-	uint8_t* genBuf = R.genVaryings.get<uint8_t>((size_t) nGen * V->varyingsSize + 512);
-	if (genBuf != H.genVaryings)
+	/* A draw seen before: its emitting half goes in behind the counts at
+	 * once, with buffers for a quarter more than its last counts; the host
+	 * checks the counts at its next library call (here, for a synchronous
+	 * framebuffer) and repeats the half with buffers for the counts in the
+	 * rare draw that did not fit (the device skipped the first attempt).  A
+	 * new draw waits for its counts. */
+	D.kind = kind;
+	D.backend = (kind == 0) ? (int) S.polygonMode : (kind == 1 ? 1 : 2);  /* 0 tiles, 1 lines, 2 points */
+	D.nIn = (uint32_t) nIn;
+	D.nTiles = H.tilesX * H.tilesY;
+	D.cacheSize = cacheSize;
+	D.varyingsSize = V->varyingsSize;
+	D.lateZ = S.lateZ;
+	D.words = H.wordPath;
+	D.fragBits = bitsFor((uint64_t) H.width * H.height + 1);
+	D.A = A;
+	D.fse = fse;
+	D.evDraw = evDraw;
+	memset(&D.rl, 0, sizeof(D.rl));
+	D.rl.params = R.dParams;
+	D.rl.nTiles = D.nTiles;
+	D.rl.lateZ = S.lateZ;
+	D.rl.words = H.wordPath;
+	D.rl.tile = (uint32_t) R.tile;
+	D.rl.batched = (R.batched && R.tile == 8) ? 1u : 0u;
+	D.rl.bigPixel = R.bigPixel;
+	/* culling kernels only where culling can act (the kernel checks the
+	 * same conditions; the other kernels need fewer registers) */
+	const bool monotone = S.depthOp == SRP_COMPARE_LESS || S.depthOp == SRP_COMPARE_LEQUAL ||
+	                      S.depthOp == SRP_COMPARE_GREATER || S.depthOp == SRP_COMPARE_GEQUAL ||
+	                      S.depthOp == SRP_COMPARE_NEVER;
+	D.rl.depthCull = (R.depthCull && S.depthTest && monotone && !S.stencilEnabled && !S.lateZ) ? 1u : 0u;
+	const uint32_t perBlock = D.rl.batched ? 1u : ((R.tile == 8) ? 4u : 1u);
+	D.rl.nBlocks = (D.nTiles + perBlock - 1) / perBlock;
+	if (D.emitted)
 	{
-		H.genVaryings = genBuf;
-		HIP_CHECK(hipMemcpyAsync(R.dParams, &H, sizeof(H), hipMemcpyHostToDevice, R.stream));
-	}
-	HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
-	A.genVaryings = H.genVaryings;
-	const uint32_t nTiles = H.tilesX * H.tilesY;
-	uint32_t* tileStart = nullptr;
-	uint32_t* tileEnd = nullptr;
-	if (backend == 0)
-	{
-		A.tris = R.tris.get<SrpTri>((size_t) nRec * sizeof(SrpTri));
-		A.pairKeys = R.pairKeys.get<uint32_t>((size_t) nItems * 4 + 16);
-		A.pairVals = R.pairVals.get<uint32_t>((size_t) nItems * 4 + 16);
-		A.pairRec = R.pairRec.get<uint32_t>((size_t) nItems * 4 + 16);
-		A.pairMask = R.pairMask.get<unsigned long long>((size_t) nItems * 8 + 16);
-		tileStart = R.tileCount.get<uint32_t>(((size_t) 2 * nTiles + 2) * 4);
-		tileEnd = tileStart + nTiles;
-		fillU32(tileStart, 0u, (size_t) 2 * nTiles, R.stream);
-		A.spans = R.spans.get<SrpSpan>((size_t) nSpans * sizeof(SrpSpan) + 64);
-		A.spanCap = nSpans;
-		A.spanCount = totals + 5;   /* emitting pass: append cursor (zeroed by the upload) */
+		/* room for a quarter more than the same draw's last counts */
+		const FitCaps& c = D.last;
+		typedef unsigned long long u64;
+		const FitCaps want = {std::min<u64>(c.records + c.records / 4 + 64, kMaxDrawRecords),
+		                      std::min<u64>(c.generated + c.generated / 4 + 64, kMaxDrawVertexSlots - cacheSize),
+		                      std::min<u64>(c.items + c.items / 4 + 4096, kMaxDrawItems), c.spans + c.spans / 4 + 64};
+		enqueueEmit(R, want, true);
 	}
 	else
 	{
-		if (backend == 1) A.lines = R.lines.get<SrpLine>((size_t) nRec * sizeof(SrpLine));
-		else A.points = R.points.get<SrpPoint>((size_t) nRec * sizeof(SrpPoint));
-		A.frags = R.frags.get<SrpFrag>((size_t) nItems * sizeof(SrpFrag) + 16);
-		A.fragKeys = R.fragKeys.get<uint32_t>((size_t) nItems * 4 + 16);
+		/* a draw not seen before waits for its counts */
+		HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
+		HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
 	}
-	launchAssemble(A, kind, true, R.stream, nClipped);
-	if (backend == 0 && nSpans)
-		launchExpandSpans(A, std::min<uint32_t>((nSpans + 3u) / 4u, 2048u), R.stream);
-
-	if (backend == 0)
-	{
-		uint32_t *keys, *vals;
-		radixSort(A.pairKeys, A.pairVals, nItems, bitsFor(nTiles), R.sort, &keys, &vals, R.stream);
-		tileBounds(keys, nItems, tileStart, tileEnd, R.stream);
-		SrpRasterLaunch rl;
-		memset(&rl, 0, sizeof(rl));
-		rl.params = R.dParams;
-		rl.tris = A.tris;
-		rl.tileList = vals;
-		rl.tileStart = tileStart;
-		rl.tileEnd = tileEnd;
-		rl.pairRec = A.pairRec;
-		rl.pairMask = A.pairMask;
-		rl.nTiles = nTiles;
-		rl.lateZ = S.lateZ;
-		rl.words = H.wordPath;
-		rl.tile = (uint32_t) R.tile;
-		rl.batched = (R.batched && R.tile == 8) ? 1u : 0u;
-		rl.bigPixel = R.bigPixel;
-		/* culling kernels only where culling can act (the kernel checks the
-		 * same conditions; the other kernels need fewer registers) */
-		const bool monotone = S.depthOp == SRP_COMPARE_LESS || S.depthOp == SRP_COMPARE_LEQUAL ||
-		                      S.depthOp == SRP_COMPARE_GREATER || S.depthOp == SRP_COMPARE_GEQUAL ||
-		                      S.depthOp == SRP_COMPARE_NEVER;
-		rl.depthCull = (R.depthCull && S.depthTest && monotone && !S.stencilEnabled && !S.lateZ) ? 1u : 0u;
-		const uint32_t perBlock = rl.batched ? 1u : ((R.tile == 8) ? 4u : 1u);
-		rl.nBlocks = (nTiles + perBlock - 1) / perBlock;
-		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
-		fse->raster(&rl, (void*) R.stream);
-		if (R.profile) R.span(0, e0, R.mark());
-	}
-	else
-	{
-		uint32_t* order = R.order.get<uint32_t>((size_t) nItems * 4 + 16);
-		iota(order, nItems, R.stream);
-		uint32_t *keys, *vals;
-		radixSort(A.fragKeys, order, nItems, bitsFor((uint64_t) H.width * H.height + 1), R.sort, &keys, &vals, R.stream);
-		SrpFragLaunch fl;
-		memset(&fl, 0, sizeof(fl));
-		fl.params = R.dParams;
-		fl.frags = A.frags;
-		fl.keys = keys;
-		fl.order = vals;
-		fl.lines = A.lines;
-		fl.points = A.points;
-		fl.count = nItems;
-		fl.kind = (backend == 1) ? 0u : 1u;
-		fl.lateZ = S.lateZ;
-		fl.words = H.wordPath;
-		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
-		fse->frags(&fl, (void*) R.stream);
-		if (R.profile) R.span(2, e0, R.mark());
-	}
-	if (R.profile)
-		R.span(1, evDraw, R.mark());
+	D.active = true;
+	if (!F->deferred || !D.emitted)
+		resolvePending();
 	if (!F->deferred)
 		fbDownload(F);
 }
@@ -1971,6 +2173,7 @@ extern "C" {
 
 SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
 {
+	resolvePending();
 	if (!cmds || n == 0)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewBatch", "No draws");
@@ -1981,6 +2184,7 @@ SRPBatch* srpNewBatch(const SRPDrawCommand* cmds, size_t n)
 
 void srpFreeBatch(SRPBatch* B)
 {
+	resolvePending();
 	if (!B)
 		return;
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
@@ -2003,6 +2207,7 @@ void srpFreeBatch(SRPBatch* B)
 
 void srpBatchRender(SRPBatch* B)
 {
+	resolvePending();
 	if (!B)
 		return;
 	Runtime& R = rt();
@@ -2029,6 +2234,7 @@ void srpBatchRender(SRPBatch* B)
 
 int srpBatchCapture(SRPBatch* B)
 {
+	resolvePending();
 	if (!B || B->cmds.empty())
 		return -1;
 	Runtime& R = rt();
@@ -2058,6 +2264,7 @@ int srpBatchCapture(SRPBatch* B)
 
 void srpBatchSetTuning(SRPBatch* B, const SRPBatchTuning* t)
 {
+	resolvePending();
 	if (!B || !t)
 		return;
 	if (t->wavesPerCU < 1 || t->wavesPerCU > 32 || t->maskBox < 0 || t->maskBox > SRP_SMALL_BBOX)
@@ -2087,6 +2294,7 @@ void srpBatchGetTuning(const SRPBatch* B, SRPBatchTuning* t)
 
 double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 {
+	resolvePending();
 	if (!B || B->cmds.empty())
 		return 0.0;
 	batchReport(B);
@@ -2136,6 +2344,7 @@ double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 
 void srpBatchInfo(const SRPBatch* B, uint64_t out[4])
 {
+	resolvePending();
 	if (!B || !out)
 		return;
 	batchReport(const_cast<SRPBatch*>(B));

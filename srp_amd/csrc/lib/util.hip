@@ -309,6 +309,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
                                                               uint32_t nBlocks)
 {
 	if (nDev && *nDev < n) n = *nDev;
+	/* blocks past the device count (a capacity-sized grid) have nothing to
+	 * place; whole blocks leave together, before any barrier */
+	if ((size_t) blockIdx.x * RS_CHUNK >= n)
+		return;
+	/* hist rows of blocks past the count are zero: not read */
+	const uint32_t usedBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
 	__shared__ uint32_t running[256];
 	__shared__ uint32_t sliceCnt[RS_SLICES * 4][256];
 	__shared__ uint32_t tileTotal[256];
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 		const uint32_t d4 = 4u * lane;
 		uint32_t tot[4] = {0u, 0u, 0u, 0u}, bef[4] = {0u, 0u, 0u, 0u};
 #pragma unroll 8
-		for (uint32_t b = wave; b < nBlocks; b += 4u)
+		for (uint32_t b = wave; b < usedBlocks; b += 4u)
 		{
 			const uint4 v = *reinterpret_cast<const uint4*>(hist + (size_t) b * 256u + d4);
 			const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
@@ -592,6 +598,22 @@ void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv,
 void iota(uint32_t* p, size_t n, hipStream_t s)
 {
 	if (n) hipLaunchKernelGGL(k_iota, fillGrid(n), dim3(256), 0, s, p, n);
+}
+
+__global__ void k_fit(const unsigned long long* totals64, const uint32_t* spanCount, FitCaps caps, uint32_t* skip,
+                      uint32_t* items)
+{
+	const bool fits = totals64[0] <= caps.records && totals64[1] <= caps.generated && totals64[2] <= caps.items &&
+	                  (unsigned long long) *spanCount <= caps.spans;
+	/* 1: does not fit; 2: nothing to emit */
+	*skip = !fits ? 1u : (totals64[0] == 0ull ? 2u : 0u);
+	*items = fits ? (uint32_t) totals64[2] : 0u;
+}
+
+void launchFit(const unsigned long long* totals64, const uint32_t* spanCount, FitCaps caps, uint32_t* skip,
+               uint32_t* items, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_fit, dim3(1), dim3(1), 0, s, totals64, spanCount, caps, skip, items);
 }
 
 }  // namespace lib

@@ -466,6 +466,55 @@ static int scCfgTris(const SceneArgs* a, SceneOut* o)
 	return scFinish(fb, o);
 }
 
+/* One draw issued again and again with counts that change between calls:
+ * p0 random clip-space triangles (seed) through vsColorMat, scaled about the
+ * origin by each of 0.05, 1, 0.3, 1, 0.5 in turn, filled, then as lines,
+ * then as points, all into one 512x512 framebuffer with depth LESS.  The
+ * GPU path sizes a draw's buffers from the same draw's previous counts and
+ * repeats its emitting half when they were too small; every growth here
+ * takes that path.  p1 = 1: in the GPU build the framebuffer stays on the
+ * device between the draws (deferred), so each draw is checked only at the
+ * next call. */
+static int scSpecGrow(const SceneArgs* a, SceneOut* o)
+{
+	const size_t nTri = a->p0 ? (size_t) a->p0 : 300;
+	VtxPosColor* v = (VtxPosColor*) malloc(3 * nTri * sizeof(VtxPosColor));
+	srpSceneBuildTris(a->seed, nTri, v);
+	UniMat u;
+	SRPVertexShader vs = scVS(sc_vsColorMat, 1, g_vColor, sizeof(VarColor));
+	SRPFragmentShader fs = scFS(sc_fsColor, false);
+	SRPShaderProgram sp = scProgram(&u, &vs, &fs);
+	SRPFramebuffer* fb = scBegin(a, 512, 512);
+	srpDepthTest(true);
+	srpDepthCompareOp(SRP_COMPARE_LESS);
+	SRPVertexBuffer* vb = srpNewVertexBuffer();
+	srpVertexBufferCopyData(vb, sizeof(VtxPosColor), 3 * nTri * sizeof(VtxPosColor), v);
+	srpFramebufferClear(fb);
+	scDepthInit(fb, 1.0f);
+#ifdef SRP_GPU_BUILD
+	if (a->p1)
+		srpFramebufferSetDeferred(fb, true);
+#endif
+	static const float scales[5] = {0.05f, 1.0f, 0.3f, 1.0f, 0.5f};
+	static const SRPPolygonMode modes[3] = {SRP_POLYGON_MODE_FILL, SRP_POLYGON_MODE_LINE, SRP_POLYGON_MODE_POINT};
+	for (int m = 0; m < 3; m++)
+	{
+		srpRasterPolygonMode(modes[m]);
+		for (int k = 0; k < 5; k++)
+		{
+			u.m = mat4ConstructScale(scales[k], scales[k], scales[k]);
+			srpDrawVertexBuffer(vb, fb, &sp, SRP_PRIM_TRIANGLES, 0, 3 * nTri);
+		}
+	}
+#ifdef SRP_GPU_BUILD
+	if (a->p1)
+		srpFramebufferSetDeferred(fb, false);
+#endif
+	srpFreeVertexBuffer(vb);
+	free(v);
+	return scFinish(fb, o);
+}
+
 /* UV sphere of p0 rings x p1 segments: (p0+1)(p1+1) vertices {pos, uv},
  * 2*p0*p1 u32-indexed triangles (degenerate at the poles). */
 void srpSceneBuildSphere(int rings, int segs, VtxPosUV** verts, size_t* nVerts, uint32_t** idx, size_t* nIdx)
@@ -722,6 +771,7 @@ static const SceneEntry kScenes[] = {
 	{"misc/scissor_test", scScissor},
 	{"misc/rasterization_rules", scRasterizationRules},
 	{"cfg/tris", scCfgTris},
+	{"spec/grow", scSpecGrow},
 	{"cfg/mesh", scCfgMesh},
 	{"cfg/mix", scCfgMix},
 	{"cfg/fuzz", srpSceneFuzz},

@@ -101,6 +101,15 @@ def test_fuzz_vs_compiled_reference(oracle, ref, seed):
     assert same(oracle.run("cfg/fuzz", seed=seed), ref.run("cfg/fuzz", seed=seed))
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_repeated_draw_vs_compiled_reference(oracle, ref, seed):
+    """One draw issued 15 times with scales and polygon modes that change
+    its counts between calls (the scene the GPU path's buffer re-sizing is
+    tested on)."""
+    c = dict(name="spec/grow", p0=300, seed=seed)
+    assert same(oracle.run(**c), ref.run(**c))
+
+
 def test_mix_vs_compiled_reference(oracle, ref):
     c = dict(name="cfg/mix", width=640, height=360, p0=60000, p1=10000, p2=10000, seed=11)
     assert same(oracle.run(**c), ref.run(**c))
