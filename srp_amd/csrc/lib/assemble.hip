@@ -608,6 +608,9 @@ __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 {
 	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
 		return;
+	if (!EMIT && A.zero)
+		for (uint32_t z = blockIdx.x * 128u + threadIdx.x; z < A.zeroCount; z += gridDim.x * 128u)
+			A.zero[z] = 0u;
 	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
 	const AsmDraw& W = A.draws[d];
 	const uint32_t k = (blockIdx.x - W.firstBlock) * 128u + threadIdx.x;

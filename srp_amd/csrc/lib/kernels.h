@@ -85,6 +85,10 @@ struct AsmArgs
 	/* emitting pass: when set and nonzero on the device, the pass does
 	 * nothing (the counted totals did not fit the capacities; see k_fit) */
 	const uint32_t* skip;
+	/* triangle counting pass: also sets zero[0..zeroCount) to 0 (the tile
+	 * ranges the emitting half fills), saving a launch */
+	uint32_t* zero;
+	uint32_t zeroCount;
 };
 
 /* kind: 0 triangles, 1 lines, 2 points.  clipped: how many triangles the
@@ -137,19 +141,23 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
                uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr);
 void radixSortReserve(SortScratch& scratch, size_t n);
 
-/* Capacities an emitting pass was sized for before its counts were known. */
+/* Capacities an emitting pass was sized for before its counts were known,
+ * and the number of triangles its clipping kernels handle (0: they were not
+ * launched, so any clipped triangle makes the draw not fit). */
 struct FitCaps
 {
-	unsigned long long records, generated, items, spans;
+	unsigned long long records, generated, items, spans, clipped;
 };
-/* Whether a counting pass's exact totals (totals64: records, generated
- * vertices, items; *spanCount: spans) fit `caps`.  One thread writes
- * *skip = 0 if they do, 1 if they do not, 2 if there is nothing to emit,
- * and *items = the item count if they fit, else 0: the emitting pass, the
- * sort and the tile bounds read these on the device, so a draw that does
- * not fit leaves every buffer unchanged. */
-void launchFit(const unsigned long long* totals64, const uint32_t* spanCount, FitCaps caps, uint32_t* skip,
-               uint32_t* items, hipStream_t s);
+/* The fit check of a single draw's counter block `totals` (u32 words: [0..2]
+ * the counting pass's totals mod 2^32, [4] clipped triangles, [6] spans,
+ * [7] the skip flag written here; [8..15] the three exact u64 totals): one
+ * thread sets totals[7] = 0 if the counts fit `caps`, 1 if they do not, 2
+ * if there is nothing to emit, and totals[2] = the item count if they fit,
+ * else 0.  The emitting pass, the sort and the tile ranges read these on
+ * the device, so a draw that does not fit leaves every buffer unchanged.
+ * With hostCopy (the device address of host-mapped memory), the 16 words
+ * are then copied there for the host. */
+void launchFit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, hipStream_t s);
 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 /* *dst = *src by a one-thread kernel (dst may be host-mapped memory) */

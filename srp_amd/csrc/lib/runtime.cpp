@@ -147,7 +147,8 @@ struct Runtime
 	SrpDrawParams* dParams = nullptr;
 	uint8_t* hUniform = nullptr;        /* pinned staging */
 	size_t uniformCap = 0;
-	uint32_t* hTotals = nullptr;        /* pinned: the draw's counters (kTotalsBytes) */
+	uint32_t* hTotals = nullptr;        /* pinned, mapped: the draw's counters (kTotalsBytes) */
+	uint32_t* dTotals = nullptr;        /* its device address */
 	unsigned long long* dStats = nullptr;
 	uint64_t hostPrims = 0, hostDraws = 0;
 	int tile = 8;
@@ -165,7 +166,8 @@ struct Runtime
 		int kind = 0, backend = 0;
 		uint32_t nIn = 0, nTiles = 0, cacheSize = 0, varyingsSize = 0, lateZ = 0, words = 0;
 		uint32_t clipHint = 0;
-		FitCaps last = {0, 0, 0, 0};   /* the same draw's counts at its last call */
+		bool clipSkipped = false;  /* counted without the clipping kernel (it clipped nothing last time) */
+		uint32_t* tileStart = nullptr;   /* tile ranges (filled triangles), zeroed by the counting pass */
 		int fragBits = 0;
 		bool emitted = false;      /* the emitting half is enqueued (else it waits for the counts) */
 		DrawKey key;
@@ -175,14 +177,9 @@ struct Runtime
 		hipEvent_t evDraw = nullptr;
 	} pending;
 	hipEvent_t countsCopied = nullptr;  /* the pending draw's counters are on the host */
-	/* each draw's counts at its last call, which size its next emitting
-	 * half, and how many triangles it clipped (the clip kernels' grids) */
-	struct DrawCounts
-	{
-		FitCaps counts;
-		uint32_t clipped;
-	};
-	std::unordered_map<DrawKey, DrawCounts, DrawKeyHash> drawCounts;
+	/* each draw's counts at its last call (clipped: triangles it clipped),
+	 * which size its next emitting half */
+	std::unordered_map<DrawKey, FitCaps, DrawKeyHash> drawCounts;
 
 	DevBuf uniform, clip, varyings, genVaryings, counts;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount, spans;
@@ -238,7 +235,8 @@ struct Runtime
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 		HIP_CHECK(hipHostMalloc((void**) &hParams, kParamsBlock, hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dParams, kParamsBlock));
-		HIP_CHECK(hipHostMalloc((void**) &hTotals, kTotalsBytes, hipHostMallocDefault));
+		HIP_CHECK(hipHostMalloc((void**) &hTotals, kTotalsBytes, hipHostMallocMapped));
+		HIP_CHECK(hipHostGetDevicePointer((void**) &dTotals, hTotals, 0));
 		HIP_CHECK(hipMalloc((void**) &dStats, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipMemset(dStats, 0, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
@@ -1190,14 +1188,15 @@ bool snapshotState(const SRPContext* C, const SRPShaderProgram* sp, int kind, Sr
  * estimate of them), the fit check of the counts against those buffers,
  * and the half itself: emitting pass, span expansion, sort, tile ranges,
  * rasteriser or fragment lists.  Speculative (the counts are not known
- * yet): the counts are also copied to the host, for resolvePending. */
+ * yet): the fit check also copies the counters to the host, for
+ * resolvePending, and want.spans or want.clipped of 0 leave out the span
+ * and clipped-triangle kernels (the check then requires none). */
 void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 {
 	Runtime::Pending& D = R.pending;
 	AsmArgs A = D.A;
 	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
-	const unsigned long long* totals64 = reinterpret_cast<const unsigned long long*>(totals + 8);
-	FitCaps caps = {kMaxDrawRecords, kMaxDrawVertexSlots - D.cacheSize, kMaxDrawItems, 0xFFFFFFFFull};
+	FitCaps caps = {kMaxDrawRecords, kMaxDrawVertexSlots - D.cacheSize, kMaxDrawItems, 0xFFFFFFFFull, want.clipped};
 	auto take = [](DevBuf& b, unsigned long long n, size_t elem, size_t pad, unsigned long long& cap) {
 		uint8_t* p = b.get<uint8_t>((size_t) n * elem + pad);
 		cap = std::min<unsigned long long>(cap, (b.cap - pad) / elem);
@@ -1218,8 +1217,8 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
 	}
 	A.genVaryings = genBuf;
-	uint32_t* tileStart = nullptr;
-	uint32_t* tileEnd = nullptr;
+	uint32_t* tileStart = D.tileStart;   /* zeroed by the counting pass */
+	uint32_t* tileEnd = tileStart ? tileStart + D.nTiles : nullptr;
 	if (D.backend == 0)
 	{
 		A.tris = (SrpTri*) take(R.tris, want.records, sizeof(SrpTri), 0, caps.records);
@@ -1228,8 +1227,6 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		A.pairRec = (uint32_t*) take(R.pairRec, want.items, 4, 16, caps.items);
 		A.pairMask = (unsigned long long*) take(R.pairMask, want.items, 8, 16, caps.items);
 		A.spans = (SrpSpan*) take(R.spans, want.spans, sizeof(SrpSpan), 64, caps.spans);
-		tileStart = R.tileCount.get<uint32_t>(((size_t) 2 * D.nTiles + 2) * 4);
-		tileEnd = tileStart + D.nTiles;
 	}
 	else
 	{
@@ -1240,26 +1237,22 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		caps.spans = 0;
 	}
 	uint32_t* order = (D.backend != 0) ? (uint32_t*) take(R.order, want.items, 4, 16, caps.items) : nullptr;
-	/* the sort, tile ranges and fragment kernel size their grids by this */
+	/* the sort, tile ranges, span and fragment kernels size their grids by these */
 	caps.items = std::min(caps.items, want.items);
+	caps.spans = std::min(caps.spans, want.spans);
 	A.recCap = (uint32_t) caps.records;
 	A.itemCap = (uint32_t) caps.items;
 	A.genCap = (uint32_t) std::min<unsigned long long>(caps.generated, 0xFFFFFFFFull);
 	A.spanCap = (uint32_t) caps.spans;
 	A.spanCount = totals + 5;   /* emitting pass: append cursor (zeroed by the upload) */
-	uint32_t* skip = totals + 7;
-	A.skip = skip;
-	uint32_t* items = totals + 2;   /* becomes the item count, or 0 when skipped */
-	launchFit(totals64, totals + 6, caps, skip, items, R.stream);
+	A.skip = totals + 7;
+	uint32_t* items = totals + 2;   /* the item count, or 0 when skipped (k_fit) */
+	launchFit(totals, caps, speculative ? R.dTotals : nullptr, R.stream);
 	if (speculative)
-	{
-		HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
 		HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
-	}
 
-	if (D.backend == 0)
-		fillU32(tileStart, 0u, (size_t) 2 * D.nTiles, R.stream);
-	launchAssemble(A, D.kind, true, R.stream, speculative ? kClipUnknown : R.hTotals[4], D.clipHint);
+	const uint32_t clipped = speculative ? (caps.clipped ? kClipUnknown : 0u) : R.hTotals[4];
+	launchAssemble(A, D.kind, true, R.stream, clipped, D.clipHint);
 	if (D.backend == 0)
 	{
 		/* spans: the grid strides over the device-side count */
@@ -1304,11 +1297,36 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		R.span(1, D.evDraw, R.mark());
 }
 
+/* The counting pass of the pending draw (triangles: with its clipping
+ * kernel unless clipSkipped) and the scans of its counts. */
+void enqueueCount(Runtime& R, bool clipSkipped)
+{
+	Runtime::Pending& D = R.pending;
+	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
+	AsmArgs A = D.A;
+	A.zero = D.tileStart;
+	A.zeroCount = D.tileStart ? 2u * D.nTiles : 0u;
+	launchAssemble(A, D.kind, false, R.stream, clipSkipped ? 0u : kClipUnknown, D.clipHint);
+	scanExclusiveMulti(A.recOff, (size_t) D.nIn + 1, 3, D.nIn, totals, R.scan, R.stream,
+	                   reinterpret_cast<unsigned long long*>(totals + 8));
+}
+
+/* Wait for the pending draw's counters on the host: copied behind the work
+ * enqueued so far. */
+void copyCounts(Runtime& R)
+{
+	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
+	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
+}
+
 /* Settle the pending draw, if any: wait for its counts (not for the draw),
  * report what they show as the synchronous path would have (clip overflow,
  * a draw over the limits: refused, nothing drawn), and when the buffers it
  * was given were too small, enqueue its emitting half again with buffers
- * for its counts (the first attempt was skipped on the device). */
+ * for its counts (the first attempt was skipped on the device).  A draw
+ * whose clipping kernel was left out but that clips triangles is counted
+ * again, with it, first. */
 void resolvePending()
 {
 	if (!g_rt || !g_rt->pending.active)
@@ -1317,8 +1335,22 @@ void resolvePending()
 	Runtime::Pending& D = R.pending;
 	D.active = false;
 	HIP_CHECK(hipEventSynchronize(R.countsCopied));
+	__atomic_thread_fence(__ATOMIC_ACQUIRE);
 	const uint32_t* h = R.hTotals;
 	const uint64_t* h64 = reinterpret_cast<const uint64_t*>(h + 8);
+	if (D.clipSkipped && h[4] != 0u)
+	{
+		/* the emitting half was skipped (k_fit): count again, with the
+		 * clipping kernel, and wait for those counts */
+		uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
+		fillU32(totals + 4, 0u, 3, R.stream);   /* clip list, span cursor, span count */
+		D.clipHint = h[4];
+		D.clipSkipped = false;
+		D.emitted = false;
+		enqueueCount(R, false);
+		copyCounts(R);
+		HIP_CHECK(hipEventSynchronize(R.countsCopied));
+	}
 	if (h[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
 		        "clipped polygon exceeded %d vertices; the primitive was dropped", SRP_MAX_POLY);
@@ -1334,10 +1366,10 @@ void resolvePending()
 		R.lastClipped = h[4];
 	R.hostDraws++;
 	R.hostPrims += h64[0];
-	const FitCaps need = {h64[0], h64[1], h64[2], h[6]};
+	const FitCaps need = {h64[0], h64[1], h64[2], h[6], h[4]};
 	if (R.drawCounts.size() >= 4096)
 		R.drawCounts.clear();
-	R.drawCounts[D.key] = {need, h[4]};
+	R.drawCounts[D.key] = need;
 	if (h64[0] != 0u && (!D.emitted || h[7] == 1u))
 		enqueueEmit(R, need, false);
 }
@@ -1548,7 +1580,8 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	/* Assembly COUNT pass + scans. */
 	uint32_t* counts = R.counts.get<uint32_t>((nIn + 1) * 4 * sizeof(uint32_t));
 	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);   /* zeroed by the upload */
-	AsmArgs A;
+	Runtime::Pending& D = R.pending;
+	AsmArgs& A = D.A;
 	memset(&A, 0, sizeof(A));
 	A.draws = reinterpret_cast<const AsmDraw*>((const uint8_t*) R.dParams + kAsmDrawOff);
 	A.nDraws = 1;
@@ -1561,24 +1594,6 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	A.clipCount = totals + 4;
 	A.spanCount = totals + 6;   /* counting pass: how many spans */
 	A.recCap = A.itemCap = A.genCap = 0xFFFFFFFFu;
-	unsigned long long* totals64 = reinterpret_cast<unsigned long long*>(totals + 8);
-	Runtime::Pending& D = R.pending;
-	D.key = {ib, vb, start, count, (uint32_t) prim, H.width, H.height};
-	auto seen = R.drawCounts.find(D.key);
-	D.emitted = seen != R.drawCounts.end();
-	D.clipHint = D.emitted ? seen->second.clipped : R.lastClipped;
-	if (D.emitted)
-		D.last = seen->second.counts;
-	launchAssemble(A, kind, false, R.stream, kClipUnknown, D.clipHint);
-	scanExclusiveMulti(A.recOff, nIn + 1, 3, nIn, totals, R.scan, R.stream, totals64);
-	if (R.profile) R.span(3, evDraw, R.mark());
-
-	/* A draw seen before: its emitting half goes in behind the counts at
-	 * once, with buffers for a quarter more than its last counts; the host
-	 * checks the counts at its next library call (here, for a synchronous
-	 * framebuffer) and repeats the half with buffers for the counts in the
-	 * rare draw that did not fit (the device skipped the first attempt).  A
-	 * new draw waits for its counts. */
 	D.kind = kind;
 	D.backend = (kind == 0) ? (int) S.polygonMode : (kind == 1 ? 1 : 2);  /* 0 tiles, 1 lines, 2 points */
 	D.nIn = (uint32_t) nIn;
@@ -1588,9 +1603,27 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	D.lateZ = S.lateZ;
 	D.words = H.wordPath;
 	D.fragBits = bitsFor((uint64_t) H.width * H.height + 1);
-	D.A = A;
 	D.fse = fse;
 	D.evDraw = evDraw;
+	D.tileStart = (D.backend == 0) ? R.tileCount.get<uint32_t>(((size_t) 2 * D.nTiles + 2) * 4) : nullptr;
+	/* A draw seen before: its emitting half goes in behind the counts at
+	 * once, with buffers for a quarter more than its last counts, and
+	 * without the span or clipping kernels when it had no spans or clipped
+	 * nothing; the host checks the counts at its next library call (here,
+	 * for a synchronous framebuffer), and in the rare draw that did not fit
+	 * (the device skipped the attempt) repeats the half for its counts.  A
+	 * new draw waits for its counts. */
+	D.key = {ib, vb, start, count, (uint32_t) prim, H.width, H.height};
+	auto seen = R.drawCounts.find(D.key);
+	D.emitted = seen != R.drawCounts.end();
+	FitCaps last = {0, 0, 0, 0, 0};
+	if (D.emitted)
+		last = seen->second;
+	D.clipHint = D.emitted ? (uint32_t) last.clipped : R.lastClipped;
+	D.clipSkipped = D.emitted && kind == 0 && last.clipped == 0;
+	enqueueCount(R, D.clipSkipped);
+	if (R.profile) R.span(3, evDraw, R.mark());
+
 	memset(&D.rl, 0, sizeof(D.rl));
 	D.rl.params = R.dParams;
 	D.rl.nTiles = D.nTiles;
@@ -1609,20 +1642,16 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	D.rl.nBlocks = (D.nTiles + perBlock - 1) / perBlock;
 	if (D.emitted)
 	{
-		/* room for a quarter more than the same draw's last counts */
-		const FitCaps& c = D.last;
 		typedef unsigned long long u64;
-		const FitCaps want = {std::min<u64>(c.records + c.records / 4 + 64, kMaxDrawRecords),
-		                      std::min<u64>(c.generated + c.generated / 4 + 64, kMaxDrawVertexSlots - cacheSize),
-		                      std::min<u64>(c.items + c.items / 4 + 4096, kMaxDrawItems), c.spans + c.spans / 4 + 64};
+		const FitCaps want = {std::min<u64>(last.records + last.records / 4 + 64, kMaxDrawRecords),
+		                      std::min<u64>(last.generated + last.generated / 4 + 64, kMaxDrawVertexSlots - cacheSize),
+		                      std::min<u64>(last.items + last.items / 4 + 4096, kMaxDrawItems),
+		                      last.spans ? last.spans + last.spans / 4 + 64 : 0ull,
+		                      last.clipped ? ~0ull : 0ull};
 		enqueueEmit(R, want, true);
 	}
 	else
-	{
-		/* a draw not seen before waits for its counts */
-		HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
-		HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
-	}
+		copyCounts(R);
 	D.active = true;
 	if (!F->deferred || !D.emitted)
 		resolvePending();

@@ -600,20 +600,22 @@ void iota(uint32_t* p, size_t n, hipStream_t s)
 	if (n) hipLaunchKernelGGL(k_iota, fillGrid(n), dim3(256), 0, s, p, n);
 }
 
-__global__ void k_fit(const unsigned long long* totals64, const uint32_t* spanCount, FitCaps caps, uint32_t* skip,
-                      uint32_t* items)
+__global__ void k_fit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy)
 {
+	const unsigned long long* totals64 = reinterpret_cast<const unsigned long long*>(totals + 8);
 	const bool fits = totals64[0] <= caps.records && totals64[1] <= caps.generated && totals64[2] <= caps.items &&
-	                  (unsigned long long) *spanCount <= caps.spans;
+	                  (unsigned long long) totals[6] <= caps.spans && (unsigned long long) totals[4] <= caps.clipped;
 	/* 1: does not fit; 2: nothing to emit */
-	*skip = !fits ? 1u : (totals64[0] == 0ull ? 2u : 0u);
-	*items = fits ? (uint32_t) totals64[2] : 0u;
+	totals[7] = !fits ? 1u : (totals64[0] == 0ull ? 2u : 0u);
+	totals[2] = fits ? (uint32_t) totals64[2] : 0u;
+	if (hostCopy)
+		for (int k = 0; k < 16; k++)
+			hostCopy[k] = totals[k];
 }
 
-void launchFit(const unsigned long long* totals64, const uint32_t* spanCount, FitCaps caps, uint32_t* skip,
-               uint32_t* items, hipStream_t s)
+void launchFit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_fit, dim3(1), dim3(1), 0, s, totals64, spanCount, caps, skip, items);
+	hipLaunchKernelGGL(k_fit, dim3(1), dim3(1), 0, s, totals, caps, hostCopy);
 }
 
 }  // namespace lib
