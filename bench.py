@@ -132,6 +132,10 @@ def gpu_frames(wl, tex, steps, warmup):
         gpu.srpBenchFrame()
     gpu.srpHipSynchronize()
     gpu.srpHipResetStats()
+    # the timed frames record events around the rasteriser only (each
+    # interval costs the stream a barrier); the other stages are timed on
+    # a few more frames afterwards
+    gpu.srpHipProfileSlots(1)
     gpu.srpHipProfileEnable(True)
     gpu.srpHipSynchronize()
     t0 = time.perf_counter()
@@ -142,8 +146,16 @@ def gpu_frames(wl, tex, steps, warmup):
     ms = (ctypes.c_double * 4)()
     launches = (ctypes.c_uint64 * 4)()
     gpu.srpHipProfileRead(ms, launches)
-    gpu.srpHipProfileEnable(False)
     st = stats(gpu, "srpHipGetStats")
+    gpu.srpHipProfileSlots(0xF)
+    gpu.srpHipProfileEnable(True)
+    staged = min(steps, 20)
+    for _ in range(staged):
+        gpu.srpBenchFrame()
+    ms2 = (ctypes.c_double * 4)()
+    launches2 = (ctypes.c_uint64 * 4)()
+    gpu.srpHipProfileRead(ms2, launches2)
+    gpu.srpHipProfileEnable(False)
     frame = read_frame(gpu, wl["w"] * wl["h"])
     gpu.srpBenchFinish()
     return {
@@ -153,9 +165,9 @@ def gpu_frames(wl, tex, steps, warmup):
         "primitives_per_frame": st[2] // steps,
         "draws_per_frame": st[3] // steps,
         "raster_ms": ms[0] / max(launches[0], 1),
-        "draw_ms": ms[1] / max(launches[1], 1),
-        "frag_ms": ms[2] / max(launches[2], 1) if launches[2] else None,
-        "front_ms": ms[3] / max(launches[3], 1),
+        "draw_ms": ms2[1] / max(launches2[1], 1),
+        "frag_ms": ms2[2] / max(launches2[2], 1) if launches2[2] else None,
+        "front_ms": ms2[3] / max(launches2[3], 1),
         "raster_launches_per_frame": launches[0] / steps,
     }, frame
 
@@ -562,6 +574,7 @@ def neural_leg(args):
                 nn.frame()
             nn.sync()
             nn.reset_stats()
+            nn.profile_slots(1)   # events around the rasteriser (and deferred shading) only
             nn.profile_enable(True)
             nn.sync()
             t0 = time.perf_counter()
@@ -571,11 +584,20 @@ def neural_leg(args):
             dt = time.perf_counter() - t0
             ms, n = nn.profile_read()
             nn.profile_enable(False)
+            nn.profile_slots(0xF)
             st = nn.stats()
             return dt / args.nn_frames * 1e3, ms[0] / max(n[0], 1), st[1] // args.nn_frames
 
         plain_ms, plain_raster, shaded = time_frames(0, None, None)
+        # the network shaded inside the rasteriser for every fragment that
+        # passes the depth test, then deferred (the default): once per
+        # visible pixel after the rasteriser
+        nn.set_deferred(0)
+        fwd_ms, fwd_raster, _ = time_frames(0, nn.neural_fs, dev)
+        nn.set_deferred(1)
         neural_ms, neural_raster, shaded_n = time_frames(0, nn.neural_fs, dev)
+        out["frame_ms_neural_in_rasteriser"] = round(fwd_ms, 4)
+        out["raster_ms_neural_in_rasteriser"] = round(fwd_raster, 4)
         # the same frame with a trivial fragment shader (capture pass 0): what
         # the rasteriser costs without shading work
         base_ms, base_raster, _ = time_frames(1, None, None)
@@ -591,8 +613,13 @@ def neural_leg(args):
         out["shaded_per_frame"] = int(shaded)
         out["neural_speedup_frame"] = round(plain_ms / neural_ms, 3)
         out["neural_speedup_raster"] = round(plain_raster / neural_raster, 3)
-        out["neural_raster_mfma_tflops"] = round(shaded_n * NN_FWD_FLOPS / (neural_raster * 1e-3) / 1e12, 2)
-        out["neural_raster_mfma_frac"] = round(shaded_n * NN_FWD_FLOPS / (neural_raster * 1e-3) / 1e12
+        # deferred, the network runs once per pixel the draw wrote: with one
+        # draw after a clear, the pixels whose depth is below 1
+        _, depth, _ = nn.read_frame(args.width, args.height)
+        resolved = int(np.count_nonzero(depth < 1.0))
+        out["resolved_pixels_per_frame"] = resolved
+        out["neural_raster_mfma_tflops"] = round(resolved * NN_FWD_FLOPS / (neural_raster * 1e-3) / 1e12, 2)
+        out["neural_raster_mfma_frac"] = round(resolved * NN_FWD_FLOPS / (neural_raster * 1e-3) / 1e12
                                                / MFMA_BF16_PEAK_TFLOPS, 5)
     finally:
         nn.free_net(net)

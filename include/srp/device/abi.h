@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 11u
+#define SRP_ABI_VERSION 12u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -193,6 +193,15 @@ typedef struct SrpRasterLaunch
 	uint32_t nSeg;
 	uint32_t xcdQueues;
 	uint32_t* queues;
+	/* Deferred shading (single draws of a shader with a wave entry and no
+	 * depth output): the batched rasteriser does every fragment operation
+	 * but shades nothing; each pixel whose colour the draw writes is
+	 * appended once, as (pixel index, triangle record of the last fragment
+	 * that wrote it), to deferList (two words per entry, room for every
+	 * pixel; *deferCount, zero at launch, counts them).  The shader's
+	 * resolve kernel then shades exactly those fragments, 64 per wave. */
+	uint32_t* deferList;
+	uint32_t* deferCount;
 } SrpRasterLaunch;
 
 typedef struct SrpFragLaunch
@@ -227,4 +236,5 @@ typedef struct SrpFsEntry
 	size_t uniformSize;
 	SrpRasterLaunchFn raster;
 	SrpFragLaunchFn frags;
+	SrpRasterLaunchFn resolve;   /* deferred shading of a raster launch's list (wave shaders), else null */
 } SrpFsEntry;

@@ -592,7 +592,11 @@ __device__ __forceinline__ uint64_t takeRound(uint64_t& rest)
 	return round;
 }
 
-template <class FS, bool LATE, bool WORDS, bool CULL>
+/* DEFER (a wave shader without depth output, single draws): fragments go
+ * through every test but are not shaded; a pixel's colour register holds
+ * the triangle record of the last fragment that wrote it, and the tile's
+ * written pixels are appended to L.deferList for srp_resolve_kernel. */
+template <class FS, bool LATE, bool WORDS, bool CULL, bool DEFER = false>
 __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, const SrpDrawParams* P, uint32_t tile,
                                                   uint32_t gtile, uint32_t& covered, uint32_t& shaded)
 {
@@ -621,6 +625,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 
 	uint32_t col = 0, sten = 0;
 	float dep = 0.0f;
+	bool wrote = false;   /* DEFER: the draw wrote this pixel's colour */
 	if (inFb)
 	{
 		col = P->color[idx];
@@ -792,7 +797,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				in.fragCoord[3] = rec;
 				in.frontFacing = front;
 				in.primitiveID = T.id;
-				if (active)
+				if (active && !DEFER)
 				{
 					if (WORDS)
 					{
@@ -811,7 +816,10 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 						interpBytes<3>(P, v, l, iw, rec, pv, bb);
 					}
 				}
-				if constexpr (FsIsWave<FS>::value)
+				if constexpr (DEFER)
+				{
+				}
+				else if constexpr (FsIsWave<FS>::value)
 				{
 					if (!active)
 						for (uint32_t q = 0; q < VaryingBuf<WORDS>::n; q++) buf[q] = 0;
@@ -836,7 +844,8 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 					if (ok)
 					{
 						if (st.stencilEnabled) sten = stencilApply(sf.passOp, sten, sf);
-						col = packColor(out.color);
+						col = DEFER ? recIdx[(head + j) & 63u] : packColor(out.color);
+						wrote = true;
 						if (st.depthTest && st.depthWrite)
 							dep = d;
 					}
@@ -927,10 +936,12 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 			typedef typename VaryingBuf<WORDS>::type BufT;
 			BufT buf[VaryingBuf<WORDS>::n];
 			in.varyings = (SRPInterpolated*) buf;
+			uint32_t fragRec = 0;   /* the fragment's triangle record */
 			if (f < nFrag)
 			{
 				const uint32_t b = fragTri[f];
-				const SrpTri& T = L.tris[recIdx[(head + b) & 63u]];
+				fragRec = recIdx[(head + b) & 63u];
+				const SrpTri& T = L.tris[fragRec];
 				facing = ((T.flags >> 3) & 1u) ? FRAG_FRONT : 0;
 				const int fx = (int) (tx * 8 + (p & 7u)), fy = (int) (ty * 8 + (p >> 3));
 				float l[3];
@@ -954,7 +965,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				}
 				if (state == FRAG_SHADED && precull && !compare<float>(st.depthOp, depth, pixDepth))
 					state = FRAG_CULLED;
-				if (state == FRAG_SHADED)
+				if (state == FRAG_SHADED && !DEFER)
 				{
 					in.uniform = (SRPUniform*) P->uniform;
 					in.fragCoord[0] = (float) ((double) fx + 0.5);
@@ -982,7 +993,10 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				}
 			}
 			const bool active = state == FRAG_SHADED;
-			if constexpr (FsIsWave<FS>::value)
+			if constexpr (DEFER)
+			{
+			}
+			else if constexpr (FsIsWave<FS>::value)
 			{
 				/* every lane calls; lanes without a fragment pass zeroed inputs */
 				if (!active)
@@ -1003,7 +1017,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				{
 					if (LATE && !isnan(out.fragDepth))
 						depth = out.fragDepth;
-					fragCol[f] = packColor(out.color);
+					fragCol[f] = DEFER ? fragRec : packColor(out.color);
 				}
 				fragDep[f] = depth;
 				fragState[f] = state | facing;
@@ -1043,6 +1057,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 				shaded++;
 			if (st.stencilEnabled) sten = stencilApply(sf.passOp, sten, sf);
 			col = fragCol[f];
+			wrote = true;
 			if (st.depthTest && st.depthWrite)
 				dep = d;
 		}
@@ -1054,21 +1069,107 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	}
 	if (inFb)
 	{
-		P->color[idx] = col;
+		if (!DEFER)
+			P->color[idx] = col;
 		P->depth[idx] = dep;
 		P->stencil[idx] = (uint8_t) sten;
 	}
+	if (DEFER)
+	{
+		/* the tile's written pixels, one list entry each (one atomic per tile) */
+		const bool app = inFb && wrote;
+		const uint64_t m = __ballot(app);
+		if (m)
+		{
+			uint32_t at = 0;
+			if (lane == 0)
+				at = atomicAdd(L.deferCount, (uint32_t) __popcll(m));
+			at = (uint32_t) __shfl((int) at, 0) + (uint32_t) __popcll(m & ltMask);
+			if (app)
+			{
+				L.deferList[2u * at] = (uint32_t) idx;
+				L.deferList[2u * at + 1u] = col;
+			}
+		}
+	}
 }
 
-template <class FS, bool LATE, bool WORDS, int WAVES, bool CULL>
+template <class FS, bool LATE, bool WORDS, int WAVES, bool CULL, bool DEFER = false>
 __global__ __launch_bounds__(64, WAVES) void srp_raster_batched(SrpRasterLaunch L)
 {
 	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
 	if (tile >= L.nTiles)
 		return;
 	uint32_t covered = 0, shaded = 0;
-	rasterTileBatched<FS, LATE, WORDS, CULL>(L, L.params, tile, tile, covered, shaded);
+	rasterTileBatched<FS, LATE, WORDS, CULL, DEFER>(L, L.params, tile, tile, covered, shaded);
 	flushStats(L.params, covered, shaded);
+}
+
+/* Deferred shading of a draw's written pixels (L.deferList, *L.deferCount
+ * entries of pixel index and triangle record): each fragment is rebuilt as
+ * the batched rasteriser built it (barycentrics by the reference's
+ * additions, depth, 1/w, the same interpolation) and shaded by the wave
+ * entry, 64 pixels per wave; the colour is written as the rasteriser would
+ * have.  Waves stride over the list. */
+template <class FS, bool WORDS>
+__global__ __launch_bounds__(256, 2) void srp_resolve_kernel(SrpRasterLaunch L)
+{
+	const SrpDrawParams* P = L.params;
+	const uint32_t n = *L.deferCount;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t pv = P->st.provokingLast ? 2u : 0u;
+	const uint32_t width = P->width;
+	for (uint32_t base = (blockIdx.x * 4u + (threadIdx.x >> 6)) * 64u; base < n; base += gridDim.x * 256u)
+	{
+		const uint32_t k = base + lane;
+		const bool active = k < n;
+		SRPFragmentShaderIn in;
+		SRPFragmentShaderOut out;
+		out.color[0] = 0.0f; out.color[1] = 0.0f; out.color[2] = 0.0f; out.color[3] = 0.0f;
+		out.fragDepth = __builtin_nanf("");
+		typedef typename VaryingBuf<WORDS>::type BufT;
+		BufT buf[VaryingBuf<WORDS>::n];
+		for (uint32_t q = 0; q < VaryingBuf<WORDS>::n; q++) buf[q] = 0;
+		in.varyings = (SRPInterpolated*) buf;
+		in.uniform = (SRPUniform*) P->uniform;
+		in.fragCoord[0] = 0.0f; in.fragCoord[1] = 0.0f; in.fragCoord[2] = 0.0f; in.fragCoord[3] = 0.0f;
+		in.frontFacing = 0;
+		in.primitiveID = 0;
+		uint32_t pix = 0;
+		if (active)
+		{
+			pix = L.deferList[2u * k];
+			const SrpTri& T = L.tris[L.deferList[2u * k + 1u]];
+			const int px = (int) (pix % width), py = (int) (pix / width);
+			float l[3];
+			baryAt(T, px, py, l);
+			const float iw[3] = {T.invW[0], T.invW[1], T.invW[2]};
+			const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
+			in.fragCoord[0] = (float) ((double) px + 0.5);
+			in.fragCoord[1] = (float) ((double) py + 0.5);
+			in.fragCoord[2] = T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2];
+			in.fragCoord[3] = rec;
+			in.frontFacing = (T.flags >> 3) & 1u;
+			in.primitiveID = T.id;
+			if (WORDS)
+			{
+				const uint32_t* v[3];
+#pragma unroll
+				for (int q = 0; q < 3; q++) v[q] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T.vref[q]));
+				interpWords<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint32_t*>(buf));
+			}
+			else
+			{
+				const uint8_t* v[3];
+#pragma unroll
+				for (int q = 0; q < 3; q++) v[q] = varyingPtr(P, T.vref[q]);
+				interpBytes<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint8_t*>(buf));
+			}
+		}
+		FS().wave(&in, &out, active);
+		if (active)
+			P->color[pix] = packColor(out.color);
+	}
 }
 
 /* Persistent multi-draw rasteriser: L.nBlocks waves take tiles of all the
@@ -1216,6 +1317,17 @@ void launchBatchedT(const SrpRasterLaunch* L, hipStream_t s)
 {
 	constexpr int W = FsIsWave<FS>::value ? 2 : SRP_RASTER_WAVES;
 	const bool cull = !LATE && L->depthCull;
+	if constexpr (FsIsWave<FS>::value && !LATE)
+		if (L->deferList && !L->nSeg)
+		{
+			/* no shading here: the registers of a plain shader's kernel */
+			constexpr int WD = SRP_RASTER_WAVES;
+			if (cull)
+				hipLaunchKernelGGL((srp_raster_batched<FS, false, WORDS, WD, true, true>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+			else
+				hipLaunchKernelGGL((srp_raster_batched<FS, false, WORDS, WD, false, true>), dim3(L->nBlocks), dim3(64), 0, s, *L);
+			return;
+		}
 	if (L->nSeg)
 	{
 		if (cull)
@@ -1261,6 +1373,19 @@ void launchRaster(const SrpRasterLaunch* L, void* stream)
 		if (L->words) launchRasterT<FS, false, true>(L, s);
 		else launchRasterT<FS, false, false>(L, s);
 	}
+}
+
+/* Deferred shading after a raster launch with a list (wave shaders only). */
+template <class FS>
+void launchResolve(const SrpRasterLaunch* L, void* stream)
+{
+	hipStream_t s = (hipStream_t) stream;
+	if (!L->deferList || L->nBlocks == 0)
+		return;
+	/* 64 pixels per wave, at most every pixel of the framebuffer listed */
+	const uint32_t blocks = L->nBlocks < 1024u ? L->nBlocks : 1024u;
+	if (L->words) hipLaunchKernelGGL((srp_resolve_kernel<FS, true>), dim3(blocks), dim3(256), 0, s, *L);
+	else hipLaunchKernelGGL((srp_resolve_kernel<FS, false>), dim3(blocks), dim3(256), 0, s, *L);
 }
 
 template <class FS>

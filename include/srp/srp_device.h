@@ -25,7 +25,12 @@
  * lanes of the batched triangle rasteriser at once, `active` false on
  * lanes without a fragment (their inputs are zeroed and their output is
  * discarded); fn serves the kernels whose lanes are not convergent (lines,
- * points, the unbatched rasteriser). */
+ * points, the unbatched rasteriser).  In single triangle draws without
+ * depth output such a shader is deferred: the rasteriser resolves
+ * visibility first, and waveFn then shades only the fragment that wins
+ * each pixel, with the same inputs it would have had (one evaluation per
+ * visible pixel instead of one per fragment that passes the depth test;
+ * the frame is the same, srpHipSetDeferredShading). */
 #pragma once
 
 #include "srp/srp.h"
@@ -54,7 +59,7 @@ extern "C" void srpHipShaderCalledOnHost(const char* name);
 	static const SrpVsEntry srpVsEntry_##symbol = {SRP_ABI_VERSION, (size_t) (uniformSize), srpVsLaunch_##symbol}; \
 	static const int srpVsReg_##symbol = (srpHipRegisterVertexShader((const void*) &symbol, &srpVsEntry_##symbol), 0);
 
-#define SRP_HIP_FS_REGISTER_(symbol, uniformSize)                                                 \
+#define SRP_HIP_FS_REGISTER_(symbol, uniformSize, resolveFn)                                      \
 	extern "C" void symbol(SRPFragmentShaderIn*, SRPFragmentShaderOut*) { srpHipShaderCalledOnHost(#symbol); } \
 	static void srpFsRaster_##symbol(const SrpRasterLaunch* l, void* s)                           \
 	{                                                                                             \
@@ -65,7 +70,7 @@ extern "C" void srpHipShaderCalledOnHost(const char* name);
 		srp::dev::launchFrags<SrpFsFunctor_##symbol>(l, s);                                       \
 	}                                                                                             \
 	static const SrpFsEntry srpFsEntry_##symbol = {SRP_ABI_VERSION, (size_t) (uniformSize),       \
-	                                               srpFsRaster_##symbol, srpFsFrags_##symbol};    \
+	                                               srpFsRaster_##symbol, srpFsFrags_##symbol, resolveFn}; \
 	static const int srpFsReg_##symbol = (srpHipRegisterFragmentShader((const void*) &symbol, &srpFsEntry_##symbol), 0);
 
 #define SRP_HIP_FRAGMENT_SHADER_WAVE(symbol, fn, waveFn, uniformSize)                             \
@@ -81,7 +86,11 @@ extern "C" void srpHipShaderCalledOnHost(const char* name);
 			waveFn(i, o, active);                                                                 \
 		}                                                                                         \
 	};                                                                                            \
-	SRP_HIP_FS_REGISTER_(symbol, uniformSize)
+	static void srpFsResolve_##symbol(const SrpRasterLaunch* l, void* s)                          \
+	{                                                                                             \
+		srp::dev::launchResolve<SrpFsFunctor_##symbol>(l, s);                                     \
+	}                                                                                             \
+	SRP_HIP_FS_REGISTER_(symbol, uniformSize, srpFsResolve_##symbol)
 
 #define SRP_HIP_FRAGMENT_SHADER(symbol, fn, uniformSize)                                          \
 	struct SrpFsFunctor_##symbol                                                                  \
@@ -91,4 +100,4 @@ extern "C" void srpHipShaderCalledOnHost(const char* name);
 			fn(i, o);                                                                             \
 		}                                                                                         \
 	};                                                                                            \
-	SRP_HIP_FS_REGISTER_(symbol, uniformSize)
+	SRP_HIP_FS_REGISTER_(symbol, uniformSize, nullptr)

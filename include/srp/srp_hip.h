@@ -64,11 +64,16 @@ void srpHipSynchronize(void);
 
 /** Optional timing with HIP events on the library stream (no host sync is
  *  added to draws): enable, then read accumulated milliseconds and counts
- *  per slot: 0 triangle rasteriser kernel, 1 whole draw (first to last GPU
- *  operation, including the wait for the host between the two passes),
- *  2 line/point fragment kernel, 3 vertex stage + assembly count pass. */
+ *  per slot: 0 triangle rasteriser kernel (with the deferred shading that
+ *  follows it), 1 whole draw (first to last GPU operation), 2 line/point
+ *  fragment kernel, 3 vertex stage + assembly count pass. */
 void srpHipProfileEnable(bool enable);
 void srpHipProfileRead(double ms[4], uint64_t launches[4]);
+/** Which slots record while profiling is on (bit s: slot s; all by
+ *  default).  Each recorded interval costs the stream two event markers,
+ *  and the device waits at each for the work before it, so timing only
+ *  the kernel of interest disturbs the rest of the frame least. */
+void srpHipProfileSlots(unsigned mask);
 
 /** Raster tuning: tile side in pixels (8 or 16). */
 void srpHipSetTileSize(int tile);
@@ -89,6 +94,17 @@ void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox);
  *  fragments (exact again with culling off). */
 void srpHipSetTileDepthCull(int enable);
 int srpHipGetTileDepthCull(void);
+/** Deferred shading (on by default) of fragment shaders registered with a
+ *  wave entry (SRP_HIP_FRAGMENT_SHADER_WAVE) that do not write depth, in
+ *  single triangle draws with the batched rasteriser: the rasteriser does
+ *  every fragment operation but shades nothing, remembering for each pixel
+ *  the fragment whose colour would be the last written; a second kernel
+ *  then evaluates the shader once per such pixel, 64 pixels per wave, on
+ *  the inputs it would have had.  The frame and the fragment counts are
+ *  the same either way; off, the shader runs inside the rasteriser for
+ *  every fragment that passes the depth test. */
+void srpHipSetDeferredShading(int enable);
+int srpHipGetDeferredShading(void);
 
 /** Shader registration (see srp/srp_device.h).  `entry` points to a
  *  struct SrpVsEntry / SrpFsEntry of srp/device/abi.h. */

@@ -148,6 +148,11 @@ struct FitCaps
 {
 	unsigned long long records, generated, items, spans, clipped;
 };
+/* 16 words of a draw's counter block copied to host-mapped memory
+ * (hostCopy: its device address), then hostCopy[16] = seq once they are
+ * visible to the host, which waits for that word. */
+void launchPublish(const uint32_t* totals, uint32_t* hostCopy, uint32_t seq, hipStream_t s);
+
 /* The fit check of a single draw's counter block `totals` (u32 words: [0..2]
  * the counting pass's totals mod 2^32, [4] clipped triangles, [6] spans,
  * [7] the skip flag written here; [8..15] the three exact u64 totals): one
@@ -155,9 +160,8 @@ struct FitCaps
  * if there is nothing to emit, and totals[2] = the item count if they fit,
  * else 0.  The emitting pass, the sort and the tile ranges read these on
  * the device, so a draw that does not fit leaves every buffer unchanged.
- * With hostCopy (the device address of host-mapped memory), the 16 words
- * are then copied there for the host. */
-void launchFit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, hipStream_t s);
+ * With hostCopy, the 16 words are then published as by launchPublish. */
+void launchFit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, uint32_t seq, hipStream_t s);
 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 /* *dst = *src by a one-thread kernel (dst may be host-mapped memory) */

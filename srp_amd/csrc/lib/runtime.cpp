@@ -13,6 +13,7 @@
  *      ordered fragment kernel (shader module). */
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -156,6 +157,7 @@ struct Runtime
 	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
 	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
 	uint32_t depthCull = 1; /* tile-level depth culling in runs */
+	uint32_t deferShading = 1;  /* wave shaders shade each pixel's visible fragment only */
 	uint32_t lastClipped = 0;   /* triangles the previous draw clipped (grid hint of a new draw) */
 
 	/* A single draw whose emitting half was enqueued before its counts
@@ -176,13 +178,14 @@ struct Runtime
 		const SrpFsEntry* fse = nullptr;
 		hipEvent_t evDraw = nullptr;
 	} pending;
-	hipEvent_t countsCopied = nullptr;  /* the pending draw's counters are on the host */
+	uint32_t countsSeq = 0;   /* publication number of the pending draw's counters (hTotals[16]) */
 	/* each draw's counts at its last call (clipped: triangles it clipped),
 	 * which size its next emitting half */
 	std::unordered_map<DrawKey, FitCaps, DrawKeyHash> drawCounts;
 
 	DevBuf uniform, clip, varyings, genVaryings, counts;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount, spans;
+	DevBuf deferList;   /* (pixel, record) of each pixel a deferred-shading draw writes */
 	ScanScratch scan;
 	SortScratch sort;
 
@@ -193,11 +196,17 @@ struct Runtime
 	 * that timing adds no host synchronisation to the draws. */
 	struct ProfRec { hipEvent_t a, b; int slot; };
 	bool profile = false;
+	uint32_t profileSlots = 0xFu;   /* slots that record events while profiling */
+	bool prof(int slot) const { return profile && ((profileSlots >> slot) & 1u); }
 	double profMs[4] = {0, 0, 0, 0};
 	uint64_t profLaunches[4] = {0, 0, 0, 0};
 	std::vector<ProfRec> profPending;
 	std::vector<hipEvent_t> eventPool;
-	hipEvent_t paramsCopied = nullptr;  /* staging block free again */
+	/* A draw's staging block is free again once its counters have been
+	 * published (they follow its uploads on the stream); a parameter patch
+	 * after that (a larger clip-vertex buffer) is waited for with this. */
+	hipEvent_t paramsCopied = nullptr;
+	bool paramsPatched = false;
 
 	hipEvent_t mark()
 	{
@@ -235,12 +244,12 @@ struct Runtime
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 		HIP_CHECK(hipHostMalloc((void**) &hParams, kParamsBlock, hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dParams, kParamsBlock));
-		HIP_CHECK(hipHostMalloc((void**) &hTotals, kTotalsBytes, hipHostMallocMapped));
+		HIP_CHECK(hipHostMalloc((void**) &hTotals, kTotalsBytes + 64, hipHostMallocMapped));
+		memset(hTotals, 0, kTotalsBytes + 64);
 		HIP_CHECK(hipHostGetDevicePointer((void**) &dTotals, hTotals, 0));
 		HIP_CHECK(hipMalloc((void**) &dStats, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipMemset(dStats, 0, kStatWords * sizeof(unsigned long long)));
 		HIP_CHECK(hipEventCreateWithFlags(&paramsCopied, hipEventDisableTiming));
-		HIP_CHECK(hipEventCreateWithFlags(&countsCopied, hipEventDisableTiming));
 		const char* t = getenv("SRP_TILE");
 		if (t && (atoi(t) == 8 || atoi(t) == 16)) tile = atoi(t);
 		const char* b = getenv("SRP_RASTER");
@@ -1024,6 +1033,12 @@ void srpHipProfileEnable(bool enable)
 	for (int i = 0; i < 4; i++) { R.profMs[i] = 0; R.profLaunches[i] = 0; }
 }
 
+void srpHipProfileSlots(unsigned mask)
+{
+	resolvePending();
+	rt().profileSlots = mask & 0xFu;
+}
+
 void srpHipProfileRead(double ms[4], uint64_t launches[4])
 {
 	resolvePending();
@@ -1083,6 +1098,13 @@ void srpHipSetTileDepthCull(int enable)
 	resolvePending();
 	rt().depthCull = enable ? 1u : 0u;
 }
+
+void srpHipSetDeferredShading(int enable)
+{
+	resolvePending();
+	rt().deferShading = enable ? 1u : 0u;
+}
+int srpHipGetDeferredShading(void) { return (int) rt().deferShading; }
 int srpHipGetTileDepthCull(void) { return (int) rt().depthCull; }
 
 }  // extern "C"
@@ -1210,11 +1232,13 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		caps.generated = std::min<unsigned long long>(caps.generated, (R.genVaryings.cap - 512) / vs);
 	if (genBuf != R.hParams->genVaryings)
 	{
-		HIP_CHECK(hipEventSynchronize(R.paramsCopied));
+		/* the block's upload itself may still be queued */
+		HIP_CHECK(hipStreamSynchronize(R.stream));
 		R.hParams->genVaryings = genBuf;
 		HIP_CHECK(hipMemcpyAsync(&R.dParams->genVaryings, &R.hParams->genVaryings, sizeof(genBuf),
 		                         hipMemcpyHostToDevice, R.stream));
 		HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
+		R.paramsPatched = true;
 	}
 	A.genVaryings = genBuf;
 	uint32_t* tileStart = D.tileStart;   /* zeroed by the counting pass */
@@ -1247,9 +1271,7 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 	A.spanCount = totals + 5;   /* emitting pass: append cursor (zeroed by the upload) */
 	A.skip = totals + 7;
 	uint32_t* items = totals + 2;   /* the item count, or 0 when skipped (k_fit) */
-	launchFit(totals, caps, speculative ? R.dTotals : nullptr, R.stream);
-	if (speculative)
-		HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
+	launchFit(totals, caps, speculative ? R.dTotals : nullptr, speculative ? ++R.countsSeq : 0u, R.stream);
 
 	const uint32_t clipped = speculative ? (caps.clipped ? kClipUnknown : 0u) : R.hTotals[4];
 	launchAssemble(A, D.kind, true, R.stream, clipped, D.clipHint);
@@ -1267,9 +1289,18 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		rl.tileEnd = tileEnd;
 		rl.pairRec = A.pairRec;
 		rl.pairMask = A.pairMask;
-		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
+		if (D.fse->resolve && !D.lateZ && rl.batched && R.deferShading)
+		{
+			/* deferred shading: a list entry per pixel at most; the counter
+			 * was zeroed by the parameter upload */
+			rl.deferList = R.deferList.get<uint32_t>((size_t) D.nTiles * 64u * 8u);
+			rl.deferCount = totals + 14;
+		}
+		hipEvent_t e0 = R.prof(0) ? R.mark() : nullptr;
 		D.fse->raster(&rl, (void*) R.stream);
-		if (R.profile) R.span(0, e0, R.mark());
+		if (rl.deferList)
+			D.fse->resolve(&rl, (void*) R.stream);
+		if (e0) R.span(0, e0, R.mark());
 	}
 	else
 	{
@@ -1289,11 +1320,11 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		fl.kind = (D.backend == 1) ? 0u : 1u;
 		fl.lateZ = D.lateZ;
 		fl.words = D.words;
-		hipEvent_t e0 = R.profile ? R.mark() : nullptr;
+		hipEvent_t e0 = R.prof(2) ? R.mark() : nullptr;
 		D.fse->frags(&fl, (void*) R.stream);
-		if (R.profile) R.span(2, e0, R.mark());
+		if (e0) R.span(2, e0, R.mark());
 	}
-	if (R.profile)
+	if (R.prof(1) && D.evDraw)
 		R.span(1, D.evDraw, R.mark());
 }
 
@@ -1311,13 +1342,39 @@ void enqueueCount(Runtime& R, bool clipSkipped)
 	                   reinterpret_cast<unsigned long long*>(totals + 8));
 }
 
-/* Wait for the pending draw's counters on the host: copied behind the work
- * enqueued so far. */
+/* Publish the pending draw's counters to the host behind the work enqueued
+ * so far (waitCounts waits for them). */
 void copyCounts(Runtime& R)
 {
 	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
-	HIP_CHECK(hipMemcpyAsync(R.hTotals, totals, kTotalsBytes, hipMemcpyDeviceToHost, R.stream));
-	HIP_CHECK(hipEventRecord(R.countsCopied, R.stream));
+	launchPublish(totals, R.dTotals, ++R.countsSeq, R.stream);
+}
+
+/* Wait until the counters of publication R.countsSeq are on the host: the
+ * publishing kernel writes them, then the number (no event, so the stream
+ * runs on without a barrier).  Should the stream drain without it, which
+ * would be a library fault, the process stops with a message rather than
+ * waiting for ever. */
+void waitCounts(Runtime& R)
+{
+	volatile uint32_t* seq = R.hTotals + 16;
+	for (uint32_t spin = 0;; spin++)
+	{
+		if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == R.countsSeq)
+			return;
+		if ((spin & 63u) == 63u)
+		{
+			const hipError_t q = hipStreamQuery(R.stream);
+			if (q == hipSuccess && __atomic_load_n(seq, __ATOMIC_ACQUIRE) != R.countsSeq)
+			{
+				fprintf(stderr, "libsrp_amd: draw counters %u were not published (last %u)\n", R.countsSeq, *seq);
+				abort();
+			}
+			if (q != hipSuccess && q != hipErrorNotReady)
+				HIP_CHECK(q);
+			sched_yield();
+		}
+	}
 }
 
 /* Settle the pending draw, if any: wait for its counts (not for the draw),
@@ -1334,8 +1391,7 @@ void resolvePending()
 	Runtime& R = *g_rt;
 	Runtime::Pending& D = R.pending;
 	D.active = false;
-	HIP_CHECK(hipEventSynchronize(R.countsCopied));
-	__atomic_thread_fence(__ATOMIC_ACQUIRE);
+	waitCounts(R);
 	const uint32_t* h = R.hTotals;
 	const uint64_t* h64 = reinterpret_cast<const uint64_t*>(h + 8);
 	if (D.clipSkipped && h[4] != 0u)
@@ -1349,7 +1405,7 @@ void resolvePending()
 		D.emitted = false;
 		enqueueCount(R, false);
 		copyCounts(R);
-		HIP_CHECK(hipEventSynchronize(R.countsCopied));
+		waitCounts(R);
 	}
 	if (h[3] & 1u)
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "clipTriangle",
@@ -1457,9 +1513,14 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		return;
 	}
 
-	/* Varying layout (interpolation.c:93-163).  The staging block may still
-	 * be read by the previous draw's last parameter copy. */
-	HIP_CHECK(hipEventSynchronize(R.paramsCopied));
+	/* Varying layout (interpolation.c:93-163).  The staging block is free:
+	 * the previous draw's counters, published after its uploads, have been
+	 * waited for (resolvePending), unless a patch followed them. */
+	if (R.paramsPatched)
+	{
+		HIP_CHECK(hipEventSynchronize(R.paramsCopied));
+		R.paramsPatched = false;
+	}
 	SrpDrawParams& H = *R.hParams;
 	memset(&H, 0, sizeof(H));
 	const SRPVertexShader* V = sp->vs;
@@ -1553,7 +1614,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		fbUpload(F);
 	else
 		fbFlushFills(F);
-	hipEvent_t evDraw = R.profile ? R.mark() : nullptr;
+	hipEvent_t evDraw = (R.prof(1) || R.prof(3)) ? R.mark() : nullptr;
 	{
 		AsmDraw& AD = *reinterpret_cast<AsmDraw*>((uint8_t*) R.hParams + kAsmDrawOff);
 		AD.P = R.dParams;
@@ -1568,7 +1629,6 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		AD.firstBlock = 0;
 	}
 	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, paramsBytes, hipMemcpyHostToDevice, R.stream));
-	HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
 
 	/* Vertex stage (single draw: the multi-draw fields stay zero). */
 	SrpVsLaunch vl;
@@ -1622,7 +1682,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	D.clipHint = D.emitted ? (uint32_t) last.clipped : R.lastClipped;
 	D.clipSkipped = D.emitted && kind == 0 && last.clipped == 0;
 	enqueueCount(R, D.clipSkipped);
-	if (R.profile) R.span(3, evDraw, R.mark());
+	if (R.prof(3) && evDraw) R.span(3, evDraw, R.mark());
 
 	memset(&D.rl, 0, sizeof(D.rl));
 	D.rl.params = R.dParams;
@@ -2256,9 +2316,9 @@ void srpBatchRender(SRPBatch* B)
 			                c.F->pub.size, R.stream);
 		return;
 	}
-	hipEvent_t e0 = R.profile ? R.mark() : nullptr;
+	hipEvent_t e0 = R.prof(1) ? R.mark() : nullptr;
 	batchLaunch(B, R.stream);
-	if (R.profile) R.span(1, e0, R.mark());
+	if (e0) R.span(1, e0, R.mark());
 }
 
 int srpBatchCapture(SRPBatch* B)

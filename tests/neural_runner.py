@@ -51,7 +51,10 @@ class Neural:
         self._get_enc = _sig(L, "srpNeuralNetGetEncoding", None, [_vp, _f32p])
         self._compare = _sig(L, "srpNeuralCompare", None, [_vp, _vp, _vp, _vp, _vp, _vp])
         self.sync = _sig(L, "srpHipSynchronize", None, [])
+        self.set_deferred = _sig(L, "srpHipSetDeferredShading", None, [_c.c_int])
+        self.get_deferred = _sig(L, "srpHipGetDeferredShading", _c.c_int, [])
         self.profile_enable = _sig(L, "srpHipProfileEnable", None, [_c.c_bool])
+        self.profile_slots = _sig(L, "srpHipProfileSlots", None, [_c.c_uint])
         self._profile_read = _sig(L, "srpHipProfileRead", None, [_vp, _vp])
         self.reset_stats = _sig(L, "srpHipResetStats", None, [])
         self._stats = _sig(L, "srpHipGetStats", None, [_vp])

@@ -277,6 +277,42 @@ def test_compare_matches_numpy(nn, w, h, covered):
             L.srpFreeFramebuffer(fb)
 
 
+@pytest.mark.parametrize("scale", [1.0, 3.0])
+def test_deferred_shading_equals_shading_in_the_rasteriser(nn, scale):
+    """The network as the terrain's fragment shader, evaluated once per
+    visible pixel after the rasteriser (deferred, the default) and for every
+    fragment that passes the depth test inside it: the same colour bits,
+    depth, stencil and fragment counts, on three poses (each network output
+    depends only on its own fragment's inputs, which the resolve rebuilds
+    exactly)."""
+    W, H = 480, 270
+    assert nn.terrain_setup(W, H, 400, 200, 256, 0) == 0
+    net = nn.new_net(7)
+    assert nn.get_deferred() == 1
+    try:
+        nn.set_params(net, (no.init_params(7) * scale).astype(np.float32))
+        nn.terrain_shader(0, nn.neural_fs, nn.net_device(net))
+        for pose in (256, 290, 333):
+            print(f"[deferred] scale {scale} pose {pose}", flush=True)
+            nn.terrain_pose(pose)
+            out = []
+            for deferred in (1, 0):
+                nn.set_deferred(deferred)
+                nn.reset_stats()
+                nn.frame()
+                out.append((nn.read_frame(W, H), nn.stats()))
+            (a, sa), (b, sb) = out
+            assert np.array_equal(a[0], b[0]), f"pose {pose}: {np.count_nonzero(a[0] != b[0])} colours differ"
+            assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+            assert np.array_equal(a[2], b[2])
+            assert sa == sb, (sa, sb)
+            assert np.count_nonzero(a[1] < 1) > W * H // 10   # the terrain covers the frame
+    finally:
+        nn.set_deferred(1)
+        nn.free_net(net)
+        nn.finish()
+
+
 def test_neural_shader_end_to_end(nn):
     """Capture -> GPU training -> the network as the terrain's fragment
     shader on held-out poses.  Coverage, depth and stencil equal the
