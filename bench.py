@@ -721,7 +721,19 @@ def batch_leg(args):
         tuned_ms, tuning = gb.autotune(3)
         out["autotune"] = {"s": round(time.perf_counter() - t0, 3), "best_ms": round(tuned_ms, 4),
                            "waves_per_cu": tuning[0], "mask_box": tuning[1], "xcd_queues": tuning[2],
-                           "pixel_runs": tuning[3], "candidates": 48}
+                           "pixel_runs": tuning[3], "tile_size": tuning[4],
+                           "waves_per_tile": 1 if tuning[4] == 8 else 4, "candidates": 56}
+        # the best of each tile size (8x8 a wave a tile; 16x16 four waves a tile)
+        by_tile = {}
+        for tile in (8, 16):
+            best = None
+            for w in (4, 8, 16, 28):
+                gb.set_tuning(w, tuning[1] if tile == 8 else 24, tuning[2], tuning[3] if tile == 8 else 1, tile)
+                ms_t, _ = timed(lambda: gb.frame(True), 3)
+                best = ms_t if best is None else min(best, ms_t)
+            by_tile[str(tile)] = round(best, 4)
+        out["autotune"]["best_ms_by_tile_size"] = by_tile
+        gb.set_tuning(*tuning)
         eager_ms, shaded = timed(lambda: gb.frame(True), 10)
         assert gb.capture() == 0
         graph_ms, g_shaded = timed(lambda: gb.frame(True), args.batch_frames)

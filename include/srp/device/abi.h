@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 12u
+#define SRP_ABI_VERSION 14u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -195,13 +195,19 @@ typedef struct SrpRasterLaunch
 	uint32_t* queues;
 	/* Deferred shading (single draws of a shader with a wave entry and no
 	 * depth output): the batched rasteriser does every fragment operation
-	 * but shades nothing; each pixel whose colour the draw writes is
-	 * appended once, as (pixel index, triangle record of the last fragment
-	 * that wrote it), to deferList (two words per entry, room for every
-	 * pixel; *deferCount, zero at launch, counts them).  The shader's
-	 * resolve kernel then shades exactly those fragments, 64 per wave. */
+	 * but shades nothing.  Tile t lists the pixels whose colour the draw
+	 * writes, as (pixel index, triangle record of the last fragment that
+	 * wrote it), in its 64 two-word slots of deferList from 64 t on, and
+	 * their number in deferCounts[t] (written for every tile).  The library
+	 * scans deferCounts in place (deferCounts[nTiles] = the total); the
+	 * shader's resolve kernels then rebuild those fragments, compacted, as
+	 * word arrays of deferStride entries in deferFrags (fragCoord x, y, z,
+	 * w; front facing; primitive ID; pixel index; then the interpolated
+	 * varyings, one array per 4-byte word) and shade them, 64 per wave. */
 	uint32_t* deferList;
-	uint32_t* deferCount;
+	uint32_t* deferCounts;
+	uint32_t* deferFrags;
+	uint32_t deferStride;
 } SrpRasterLaunch;
 
 typedef struct SrpFragLaunch

@@ -361,17 +361,14 @@ __device__ __forceinline__ uint32_t xcdOrder(uint32_t b, uint32_t nBlocks)
  * primitive list in primitive order (so per-pixel API order holds).  The
  * barycentrics of a lane are replayed from the bbox origin exactly as the
  * reference's scan would have produced them. */
+/* One tile of that kernel: `tile` is the tile of draw P, `gtile` its entry
+ * in the tile ranges, `local` this lane's pixel of the tile; the counts
+ * accumulate into `covered` / `shaded`. */
 template <class FS, bool LATE, bool WORDS, int TILE>
-__global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
+__device__ __forceinline__ void rasterTileSimple(const SrpRasterLaunch& L, const SrpDrawParams* P, uint32_t tile,
+                                                 uint32_t gtile, uint32_t local, uint32_t& covered, uint32_t& shaded)
 {
-	constexpr int TILES_PER_BLOCK = (TILE == 8) ? 4 : 1;
-	const SrpDrawParams* P = L.params;
-	const uint32_t t = threadIdx.x;
-	const uint32_t local = (TILE == 8) ? (t & 63u) : t;
-	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks) * TILES_PER_BLOCK + ((TILE == 8) ? (t >> 6) : 0u);
-	if (tile >= L.nTiles)
-		return;
-	const uint32_t begin = L.tileStart[tile], end = L.tileEnd[tile];
+	const uint32_t begin = L.tileStart[gtile], end = L.tileEnd[gtile];
 	if (begin == end)
 		return;
 	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;
@@ -392,7 +389,10 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
 
 	for (uint32_t i = begin; i < end; i++)
 	{
-		const SrpTri* T = &L.tris[L.pairRec[L.tileList[i]] & 0x7FFFFFFFu];
+		const uint32_t pr = L.pairRec[L.tileList[i]];
+		if (pr == 0xFFFFFFFFu)   /* a pair without a record (a batch's capacity exceeded) covers nothing */
+			continue;
+		const SrpTri* T = &L.tris[pr & 0x7FFFFFFFu];
 		if (!(px >= (int) T->x0 && px < (int) T->x1 && py >= (int) T->y0 && py < (int) T->y1))
 			continue;
 		const int ny = py - (int) T->y0, nx = px - (int) T->x0;
@@ -442,7 +442,65 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
 		P->depth[idx] = ps.depth;
 		P->stencil[idx] = (uint8_t) ps.stencil;
 	}
-	flushStats(P, ps.covered, ps.shaded);
+	covered += ps.covered;
+	shaded += ps.shaded;
+}
+
+template <class FS, bool LATE, bool WORDS, int TILE>
+__global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
+{
+	constexpr int TILES_PER_BLOCK = (TILE == 8) ? 4 : 1;
+	const uint32_t t = threadIdx.x;
+	const uint32_t local = (TILE == 8) ? (t & 63u) : t;
+	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks) * TILES_PER_BLOCK + ((TILE == 8) ? (t >> 6) : 0u);
+	if (tile >= L.nTiles)
+		return;
+	uint32_t covered = 0, shaded = 0;
+	rasterTileSimple<FS, LATE, WORDS, TILE>(L, L.params, tile, tile, local, covered, shaded);
+	flushStats(L.params, covered, shaded);
+}
+
+/* Persistent multi-draw form of srp_raster_kernel for 16x16 tiles (four
+ * waves a tile, a lane a pixel): blocks take tiles of all the draws from
+ * the device queues as srp_raster_persistent does, the whole block one
+ * tile at a time. */
+template <class FS, bool LATE, bool WORDS>
+__global__ __launch_bounds__(256) void srp_raster_persistent16(SrpRasterLaunch L)
+{
+	__shared__ uint32_t taken;
+	const uint32_t nq = L.xcdQueues ? 8u : 1u;
+	const uint32_t q0 = blockIdx.x % nq;
+	uint32_t covered = 0, shaded = 0;
+	const SrpDrawParams* last = nullptr;
+	for (uint32_t k = 0; k < nq; k++)
+	{
+		const uint32_t q = (q0 + k) % nq;
+		const uint32_t lo = (uint32_t) (((uint64_t) L.nTiles * q) / nq);
+		const uint32_t hi = (uint32_t) (((uint64_t) L.nTiles * (q + 1u)) / nq);
+		for (;;)
+		{
+			if (threadIdx.x == 0)
+				taken = atomicAdd(&L.queues[q], 1u) + lo;
+			__syncthreads();
+			const uint32_t t = taken;
+			__syncthreads();   /* every lane has read it before the next take */
+			if (t >= hi)
+				break;
+			uint32_t a = 0, b = L.nSeg;
+			while (b - a > 1)
+			{
+				const uint32_t m = (a + b) >> 1;
+				if (L.seg[m].first <= t) a = m;
+				else b = m;
+			}
+			const SrpSeg sg = L.seg[a];
+			rasterTileSimple<FS, LATE, WORDS, 16>(L, sg.P, t - sg.first, t, threadIdx.x, covered, shaded);
+			last = sg.P;
+		}
+	}
+	/* every draw of a batch shares one counter array */
+	if (last)
+		flushStats(last, covered, shaded);
 }
 
 /* Batched tile rasteriser (one 64-lane wave per 8x8 tile).  A long tile
@@ -614,7 +672,11 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	const uint32_t lane = threadIdx.x;
 	const uint32_t begin = L.tileStart[gtile], end = L.tileEnd[gtile];
 	if (begin == end)
+	{
+		if (DEFER && lane == 0)
+			L.deferCounts[gtile] = 0;
 		return;
+	}
 	const uint32_t tx = tile % P->tilesX, ty = tile / P->tilesX;
 	const int px = (int) (tx * 8 + (lane & 7u));
 	const int py = (int) (ty * 8 + (lane >> 3));
@@ -1076,20 +1138,16 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 	}
 	if (DEFER)
 	{
-		/* the tile's written pixels, one list entry each (one atomic per tile) */
+		/* the tile's written pixels in its own slots, and their number */
 		const bool app = inFb && wrote;
 		const uint64_t m = __ballot(app);
-		if (m)
+		if (lane == 0)
+			L.deferCounts[gtile] = (uint32_t) __popcll(m);
+		if (app)
 		{
-			uint32_t at = 0;
-			if (lane == 0)
-				at = atomicAdd(L.deferCount, (uint32_t) __popcll(m));
-			at = (uint32_t) __shfl((int) at, 0) + (uint32_t) __popcll(m & ltMask);
-			if (app)
-			{
-				L.deferList[2u * at] = (uint32_t) idx;
-				L.deferList[2u * at + 1u] = col;
-			}
+			const size_t at = (size_t) gtile * 64u + (size_t) __popcll(m & ltMask);
+			L.deferList[2u * at] = (uint32_t) idx;
+			L.deferList[2u * at + 1u] = col;
 		}
 	}
 }
@@ -1105,20 +1163,87 @@ __global__ __launch_bounds__(64, WAVES) void srp_raster_batched(SrpRasterLaunch 
 	flushStats(L.params, covered, shaded);
 }
 
-/* Deferred shading of a draw's written pixels (L.deferList, *L.deferCount
- * entries of pixel index and triangle record): each fragment is rebuilt as
- * the batched rasteriser built it (barycentrics by the reference's
- * additions, depth, 1/w, the same interpolation) and shaded by the wave
- * entry, 64 pixels per wave; the colour is written as the rasteriser would
- * have.  Waves stride over the list. */
-template <class FS, bool WORDS>
-__global__ __launch_bounds__(256, 2) void srp_resolve_kernel(SrpRasterLaunch L)
+/* Deferred shading of a draw's written pixels (the tiles' slots of
+ * L.deferList, pixel index and triangle record, counted and then offset by
+ * L.deferCounts), in two kernels so that each runs at the occupancy it
+ * needs:
+ *   srp_defer_gather  rebuilds each listed pixel's fragment as the batched
+ *                     rasteriser built it (barycentrics by the reference's
+ *                     additions, depth, 1/w, the same interpolation) into
+ *                     the word arrays of L.deferFrags (many waves, few
+ *                     registers: the dependent loads overlap);
+ *   srp_defer_shade   shades them with the wave entry, 64 per wave, and
+ *                     writes the colour as the rasteriser would have. */
+__device__ __forceinline__ uint32_t deferWords(const SrpDrawParams* P, bool words)
+{
+	return words ? P->nWords : (P->describedSize + 3u) / 4u;
+}
+
+template <bool WORDS>
+__global__ __launch_bounds__(256) void srp_defer_gather(SrpRasterLaunch L)
 {
 	const SrpDrawParams* P = L.params;
-	const uint32_t n = *L.deferCount;
-	const uint32_t lane = threadIdx.x & 63u;
 	const uint32_t pv = P->st.provokingLast ? 2u : 0u;
 	const uint32_t width = P->width;
+	const size_t S = L.deferStride;
+	const uint32_t nw = deferWords(P, WORDS);
+	uint32_t* F = L.deferFrags;
+	const uint32_t slots = L.nTiles * 64u;
+	for (uint32_t s = blockIdx.x * 256u + threadIdx.x; s < slots; s += gridDim.x * 256u)
+	{
+		/* slot r of tile t holds entry off[t] + r of the compacted list */
+		const uint32_t t = s >> 6, r = s & 63u;
+		const uint32_t off = L.deferCounts[t];
+		if (r >= L.deferCounts[t + 1] - off)
+			continue;
+		const uint32_t k = off + r;
+		const uint32_t pix = L.deferList[2u * s];
+		const SrpTri& T = L.tris[L.deferList[2u * s + 1u]];
+		const int px = (int) (pix % width), py = (int) (pix / width);
+		float l[3];
+		baryAt(T, px, py, l);
+		const float iw[3] = {T.invW[0], T.invW[1], T.invW[2]};
+		const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
+		F[0 * S + k] = __float_as_uint((float) ((double) px + 0.5));
+		F[1 * S + k] = __float_as_uint((float) ((double) py + 0.5));
+		F[2 * S + k] = __float_as_uint(T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2]);
+		F[3 * S + k] = __float_as_uint(rec);
+		F[4 * S + k] = (T.flags >> 3) & 1u;
+		F[5 * S + k] = T.id;
+		F[6 * S + k] = pix;
+		if (WORDS)
+		{
+			const uint32_t* v[3];
+#pragma unroll
+			for (int q = 0; q < 3; q++) v[q] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T.vref[q]));
+			uint32_t buf[SRP_MAX_WORDS];
+			interpWords<3>(P, v, l, iw, rec, pv, buf);
+#pragma unroll
+			for (uint32_t w = 0; w < SRP_MAX_WORDS; w++)
+				if (w < nw) F[(7 + w) * S + k] = buf[w];
+		}
+		else
+		{
+			const uint8_t* v[3];
+#pragma unroll
+			for (int q = 0; q < 3; q++) v[q] = varyingPtr(P, T.vref[q]);
+			uint32_t buf[SRP_MAX_VARYING_BYTES / 4];
+			for (uint32_t w = 0; w < nw; w++) buf[w] = 0;
+			interpBytes<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint8_t*>(buf));
+			for (uint32_t w = 0; w < nw; w++) F[(7 + w) * S + k] = buf[w];
+		}
+	}
+}
+
+template <class FS, bool WORDS>
+__global__ __launch_bounds__(256, 2) void srp_defer_shade(SrpRasterLaunch L)
+{
+	const SrpDrawParams* P = L.params;
+	const uint32_t n = L.deferCounts[L.nTiles];
+	const uint32_t lane = threadIdx.x & 63u;
+	const size_t S = L.deferStride;
+	const uint32_t nw = deferWords(P, WORDS);
+	const uint32_t* F = L.deferFrags;
 	for (uint32_t base = (blockIdx.x * 4u + (threadIdx.x >> 6)) * 64u; base < n; base += gridDim.x * 256u)
 	{
 		const uint32_t k = base + lane;
@@ -1129,46 +1254,19 @@ __global__ __launch_bounds__(256, 2) void srp_resolve_kernel(SrpRasterLaunch L)
 		out.fragDepth = __builtin_nanf("");
 		typedef typename VaryingBuf<WORDS>::type BufT;
 		BufT buf[VaryingBuf<WORDS>::n];
-		for (uint32_t q = 0; q < VaryingBuf<WORDS>::n; q++) buf[q] = 0;
+		for (uint32_t q = 0; q < VaryingBuf<WORDS>::n; q++)
+			buf[q] = (active && q < nw) ? F[(7 + q) * S + k] : 0u;
 		in.varyings = (SRPInterpolated*) buf;
 		in.uniform = (SRPUniform*) P->uniform;
-		in.fragCoord[0] = 0.0f; in.fragCoord[1] = 0.0f; in.fragCoord[2] = 0.0f; in.fragCoord[3] = 0.0f;
-		in.frontFacing = 0;
-		in.primitiveID = 0;
-		uint32_t pix = 0;
-		if (active)
-		{
-			pix = L.deferList[2u * k];
-			const SrpTri& T = L.tris[L.deferList[2u * k + 1u]];
-			const int px = (int) (pix % width), py = (int) (pix / width);
-			float l[3];
-			baryAt(T, px, py, l);
-			const float iw[3] = {T.invW[0], T.invW[1], T.invW[2]};
-			const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
-			in.fragCoord[0] = (float) ((double) px + 0.5);
-			in.fragCoord[1] = (float) ((double) py + 0.5);
-			in.fragCoord[2] = T.zw[0] * l[0] + T.zw[1] * l[1] + T.zw[2] * l[2];
-			in.fragCoord[3] = rec;
-			in.frontFacing = (T.flags >> 3) & 1u;
-			in.primitiveID = T.id;
-			if (WORDS)
-			{
-				const uint32_t* v[3];
-#pragma unroll
-				for (int q = 0; q < 3; q++) v[q] = reinterpret_cast<const uint32_t*>(varyingPtr(P, T.vref[q]));
-				interpWords<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint32_t*>(buf));
-			}
-			else
-			{
-				const uint8_t* v[3];
-#pragma unroll
-				for (int q = 0; q < 3; q++) v[q] = varyingPtr(P, T.vref[q]);
-				interpBytes<3>(P, v, l, iw, rec, pv, reinterpret_cast<uint8_t*>(buf));
-			}
-		}
+		in.fragCoord[0] = active ? __uint_as_float(F[0 * S + k]) : 0.0f;
+		in.fragCoord[1] = active ? __uint_as_float(F[1 * S + k]) : 0.0f;
+		in.fragCoord[2] = active ? __uint_as_float(F[2 * S + k]) : 0.0f;
+		in.fragCoord[3] = active ? __uint_as_float(F[3 * S + k]) : 0.0f;
+		in.frontFacing = active ? F[4 * S + k] : 0u;
+		in.primitiveID = active ? F[5 * S + k] : 0u;
 		FS().wave(&in, &out, active);
 		if (active)
-			P->color[pix] = packColor(out.color);
+			P->color[F[6 * S + k]] = packColor(out.color);
 	}
 }
 
@@ -1304,6 +1402,12 @@ void launchVS(const SrpVsLaunch* L, void* stream)
 template <class FS, bool LATE, bool WORDS>
 void launchRasterT(const SrpRasterLaunch* L, hipStream_t s)
 {
+	if (L->nSeg)
+	{
+		/* batches: 16x16 tiles (8x8 ones use the batched kernel) */
+		hipLaunchKernelGGL((srp_raster_persistent16<FS, LATE, WORDS>), dim3(L->nBlocks), dim3(256), 0, s, *L);
+		return;
+	}
 	if (L->tile == 16)
 		hipLaunchKernelGGL((srp_raster_kernel<FS, LATE, WORDS, 16>), dim3(L->nBlocks), dim3(256), 0, s, *L);
 	else
@@ -1375,17 +1479,27 @@ void launchRaster(const SrpRasterLaunch* L, void* stream)
 	}
 }
 
-/* Deferred shading after a raster launch with a list (wave shaders only). */
+/* Deferred shading after a raster launch with a list (wave shaders only):
+ * grids sized for every pixel of the framebuffer listed (the kernels stride
+ * over the device-side count). */
 template <class FS>
 void launchResolve(const SrpRasterLaunch* L, void* stream)
 {
 	hipStream_t s = (hipStream_t) stream;
-	if (!L->deferList || L->nBlocks == 0)
+	if (!L->deferList || !L->deferCounts || !L->deferFrags || L->nTiles == 0)
 		return;
-	/* 64 pixels per wave, at most every pixel of the framebuffer listed */
-	const uint32_t blocks = L->nBlocks < 1024u ? L->nBlocks : 1024u;
-	if (L->words) hipLaunchKernelGGL((srp_resolve_kernel<FS, true>), dim3(blocks), dim3(256), 0, s, *L);
-	else hipLaunchKernelGGL((srp_resolve_kernel<FS, false>), dim3(blocks), dim3(256), 0, s, *L);
+	const uint32_t gather = L->nTiles / 4u + 1u < 4096u ? L->nTiles / 4u + 1u : 4096u;
+	const uint32_t shade = L->nBlocks < 1024u ? L->nBlocks : 1024u;
+	if (L->words)
+	{
+		hipLaunchKernelGGL((srp_defer_gather<true>), dim3(gather), dim3(256), 0, s, *L);
+		hipLaunchKernelGGL((srp_defer_shade<FS, true>), dim3(shade), dim3(256), 0, s, *L);
+	}
+	else
+	{
+		hipLaunchKernelGGL((srp_defer_gather<false>), dim3(gather), dim3(256), 0, s, *L);
+		hipLaunchKernelGGL((srp_defer_shade<FS, false>), dim3(shade), dim3(256), 0, s, *L);
+	}
 }
 
 template <class FS>

@@ -142,6 +142,8 @@ typedef struct SRPBatchTuning
 	int xcdQueues;    /**< 1: one tile queue per XCD range (work stealing between them); 0: one queue */
 	int pixelRuns;    /**< 1: triangles without exact masks are shaded pixel-parallel in runs; 0: via fragment lists */
 	int depthCull;    /**< 1: tile-level depth culling (see srpHipSetTileDepthCull; covered counts become a lower bound) */
+	int tileSize;     /**< raster tile side: 8 (one wave a tile, the batched kernel) or 16 (four waves a tile, a lane
+	                       a pixel; no binning masks or pixel runs) */
 } SRPBatchTuning;
 
 typedef struct SRPBatch SRPBatch;
@@ -169,10 +171,11 @@ int srpBatchCapture(SRPBatch* batch);
 /** Change the tuning (drops a captured graph) / read it. */
 void srpBatchSetTuning(SRPBatch* batch, const SRPBatchTuning* tuning);
 void srpBatchGetTuning(const SRPBatch* batch, SRPBatchTuning* tuning);
-/** Time `trials` frames of every candidate tuning (48: waves per CU 4, 8,
- *  16, 28 x mask box 0, 16, 24 x both queue modes x both run modes) and keep
- *  the fastest;
- *  returns its mean frame time in ms and writes it to *best (if not NULL). */
+/** Time `trials` frames of every candidate tuning and keep the fastest: 48
+ *  with 8x8 tiles (a wave a tile: waves per CU 4, 8, 16, 28 x mask box 0,
+ *  16, 24 x both queue modes x both run modes) and 8 with 16x16 tiles (four
+ *  waves a tile: waves per CU x queue modes); returns its mean frame time in
+ *  ms and writes it to *best (if not NULL). */
 double srpBatchAutotune(SRPBatch* batch, int trials, SRPBatchTuning* best);
 /** draws, assembled primitives, binned (tile, triangle) pairs, tiles of the last frame sizing */
 void srpBatchInfo(const SRPBatch* batch, uint64_t out[4]);

@@ -35,7 +35,7 @@
 #define SRP_INCLUDE_VEC
 #define SRP_INCLUDE_MAT
 #include "srp/srp.h"
-#include "png_decode.h"
+#include "image_decode.h"
 #include "png_encode.h"
 
 /* vec/mat helpers: same definitions the library exports. */
@@ -282,7 +282,7 @@ SRPTexture* srpNewTexture(const char* path, SRPTextureWrappingMode wx, SRPTextur
 {
 	SRPTexture* t = (SRPTexture*) malloc(sizeof(SRPTexture));
 	const char* err = NULL;
-	t->rgb = srpPngLoadRGB(path, &t->w, &t->h, &err);
+	t->rgb = srpImageLoadRGB(path, &t->w, &t->h, &err);
 	if (t->rgb == NULL)
 	{
 		orcMessage(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewTexture", "Failed to load image `%s`: %s", path, err);

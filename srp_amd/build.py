@@ -64,7 +64,7 @@ def build_library(jobs=8):
     os.makedirs(OBJ, exist_ok=True)
     inc = [INCLUDE, os.path.join(CSRC, "lib"), os.path.join(CSRC, "host")]
     hip_srcs = ["lib/runtime.cpp", "lib/assemble.hip", "lib/util.hip", "neural/mlp.hip"]
-    c_srcs = ["host/math.c", "host/png_decode.c", "host/png_encode.c", "host/obj_load.c"]
+    c_srcs = ["host/math.c", "host/png_decode.c", "host/image_decode.c", "host/png_encode.c", "host/obj_load.c"]
     tasks = []
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for s in hip_srcs:
@@ -149,6 +149,7 @@ def build_integration_sample():
     _run([CC, "-std=c11", "-O2", "-Wall", "-Wno-unused-function"] + NUMERIC_FLAGS + inc + ["-I", host] +
          [os.path.join(src, "main.c"), os.path.join(INTEGRATION_DIR, "host_shaders.c"),
           os.path.join(REPO, "oracle", "srp_oracle.c"), os.path.join(host, "png_decode.c"),
+          os.path.join(host, "image_decode.c"),
           os.path.join(host, "png_encode.c"), "-lz", "-lm", "-o", os.path.join(INTEGRATION_BUILD, "sample_oracle")])
     return os.path.join(INTEGRATION_BUILD, "sample_gpu")
 

@@ -31,7 +31,6 @@ static uint8_t* fail(const char** err, const char* msg, void* a, void* b)
 
 uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** err)
 {
-	static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
 	FILE* f = fopen(path, "rb");
 	if (!f) return fail(err, "can't fopen", NULL, NULL);
 	fseek(f, 0, SEEK_END);
@@ -41,7 +40,15 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 	uint8_t* file = (uint8_t*) malloc((size_t) fsize);
 	if (!file || fread(file, 1, (size_t) fsize, f) != (size_t) fsize) { fclose(f); return fail(err, "read error", file, NULL); }
 	fclose(f);
-	if (memcmp(file, sig, 8) != 0) return fail(err, "not a PNG file", file, NULL);
+	uint8_t* out = srpPngDecodeRGB(file, (size_t) fsize, width, height, err);
+	free(file);
+	return out;
+}
+
+uint8_t* srpPngDecodeRGB(const uint8_t* file, size_t fsize, int* width, int* height, const char** err)
+{
+	static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+	if (fsize < 8 || memcmp(file, sig, 8) != 0) return fail(err, "not a PNG file", NULL, NULL);
 
 	uint32_t w = 0, h = 0;
 	int depth = 0, ctype = -1, interlace = 0;
@@ -56,7 +63,7 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 		const uint32_t len = be32(file + pos);
 		const uint8_t* type = file + pos + 4;
 		const uint8_t* data = file + pos + 8;
-		if (pos + 12 + (size_t) len > (size_t) fsize) return fail(err, "truncated chunk", file, idat);
+		if (pos + 12 + (size_t) len > (size_t) fsize) return fail(err, "truncated chunk", NULL, idat);
 		if (!memcmp(type, "IHDR", 4) && len >= 13)
 		{
 			w = be32(data); h = be32(data + 4);
@@ -70,7 +77,7 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 			{
 				idatCap = (idatLen + len) * 2;
 				uint8_t* grown = (uint8_t*) realloc(idat, idatCap);
-				if (!grown) return fail(err, "out of memory", file, idat);
+				if (!grown) return fail(err, "out of memory", NULL, idat);
 				idat = grown;
 			}
 			memcpy(idat + idatLen, data, len);
@@ -80,11 +87,11 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 			break;
 		pos += 12 + (size_t) len;
 	}
-	if (w == 0 || h == 0 || ctype < 0) return fail(err, "missing IHDR", file, idat);
+	if (w == 0 || h == 0 || ctype < 0) return fail(err, "missing IHDR", NULL, idat);
 	/* as stb_image (the reference's loader): sides above 2^24 are refused,
 	 * so no size product below can overflow or truncate */
-	if (w > (1u << 24) || h > (1u << 24)) return fail(err, "image too large", file, idat);
-	if (interlace) return fail(err, "interlaced PNG not supported", file, idat);
+	if (w > (1u << 24) || h > (1u << 24)) return fail(err, "image too large", NULL, idat);
+	if (interlace) return fail(err, "interlaced PNG not supported", NULL, idat);
 
 	int channels;
 	switch (ctype)
@@ -94,10 +101,10 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 		case 3: channels = 1; break;
 		case 4: channels = 2; break;
 		case 6: channels = 4; break;
-		default: return fail(err, "bad colour type", file, idat);
+		default: return fail(err, "bad colour type", NULL, idat);
 	}
 	if (!(depth == 8 || depth == 16 || ((ctype == 0 || ctype == 3) && (depth == 1 || depth == 2 || depth == 4))))
-		return fail(err, "unsupported bit depth", file, idat);
+		return fail(err, "unsupported bit depth", NULL, idat);
 
 	const size_t bitsPerPixel = (size_t) channels * (size_t) depth;
 	const size_t rowBytes = ((size_t) w * bitsPerPixel + 7) / 8;
@@ -106,17 +113,16 @@ uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** e
 	 * w * h * 3 stay below 2^55; zlib's length type must hold the former,
 	 * and the output's side lengths must fit an int */
 	if ((rowBytes + 1) * (size_t) h > (size_t) (uLongf) -1 || (size_t) w * h * 3 > ((size_t) 1 << 48))
-		return fail(err, "image too large", file, idat);
+		return fail(err, "image too large", NULL, idat);
 	uLongf rawLen = (uLongf) ((rowBytes + 1) * h);
 	uint8_t* raw = (uint8_t*) malloc(rawLen);
-	if (!raw) return fail(err, "out of memory", file, idat);
+	if (!raw) return fail(err, "out of memory", NULL, idat);
 	if (uncompress(raw, &rawLen, idat, (uLong) idatLen) != Z_OK || rawLen != (rowBytes + 1) * h)
 	{
 		free(raw);
-		return fail(err, "corrupt image data", file, idat);
+		return fail(err, "corrupt image data", NULL, idat);
 	}
 	free(idat);
-	free(file);
 
 	/* Undo the per-scanline filters in place. */
 	for (size_t y = 0; y < h; y++)

@@ -18,6 +18,8 @@ extern "C" {
 /** Decodes `path` into a malloc'ed RGB8 buffer (row-major, 3 bytes/texel).
  *  Returns NULL on failure and points *err at a static message. */
 uint8_t* srpPngLoadRGB(const char* path, int* width, int* height, const char** err);
+/** The same from a PNG file's bytes in memory (not taken over). */
+uint8_t* srpPngDecodeRGB(const uint8_t* data, size_t size, int* width, int* height, const char** err);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@
 #include "srp/device/abi.h"
 #include "kernels.h"
 #include "sizing.h"
-#include "../host/png_decode.h"
+#include "../host/image_decode.h"
 #include "../host/png_encode.h"
 
 using namespace srp::lib;
@@ -168,6 +168,7 @@ struct Runtime
 		int kind = 0, backend = 0;
 		uint32_t nIn = 0, nTiles = 0, cacheSize = 0, varyingsSize = 0, lateZ = 0, words = 0;
 		uint32_t clipHint = 0;
+		uint32_t fragWords = 0;    /* 4-byte words of a fragment's interpolated varyings */
 		bool clipSkipped = false;  /* counted without the clipping kernel (it clipped nothing last time) */
 		uint32_t* tileStart = nullptr;   /* tile ranges (filled triangles), zeroed by the counting pass */
 		int fragBits = 0;
@@ -186,6 +187,8 @@ struct Runtime
 	DevBuf uniform, clip, varyings, genVaryings, counts;
 	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount, spans;
 	DevBuf deferList;   /* (pixel, record) of each pixel a deferred-shading draw writes */
+	DevBuf deferCounts; /* pixels each tile lists there, then their offsets */
+	DevBuf deferFrags;  /* those pixels' fragments, rebuilt for shading */
 	ScanScratch scan;
 	SortScratch sort;
 
@@ -865,7 +868,7 @@ SRPTexture* srpNewTexture(const char* path, SRPTextureWrappingMode wx, SRPTextur
 {
 	int w = 0, h = 0;
 	const char* err = "unknown error";
-	uint8_t* rgb = srpPngLoadRGB(path, &w, &h, &err);
+	uint8_t* rgb = srpImageLoadRGB(path, &w, &h, &err);
 	if (!rgb)
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpNewTexture", "Failed to load image `%s`: %s", path, err);
@@ -1294,12 +1297,19 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 			/* deferred shading: a list entry per pixel at most; the counter
 			 * was zeroed by the parameter upload */
 			rl.deferList = R.deferList.get<uint32_t>((size_t) D.nTiles * 64u * 8u);
-			rl.deferCount = totals + 14;
+			rl.deferCounts = R.deferCounts.get<uint32_t>(((size_t) D.nTiles + 1u) * 4u);
+			/* the rebuilt fragments: 7 words plus the varyings' words */
+			const size_t words = 7u + D.fragWords;
+			rl.deferStride = D.nTiles * 64u;
+			rl.deferFrags = R.deferFrags.get<uint32_t>(words * rl.deferStride * 4u);
 		}
 		hipEvent_t e0 = R.prof(0) ? R.mark() : nullptr;
 		D.fse->raster(&rl, (void*) R.stream);
 		if (rl.deferList)
+		{
+			scanExclusive(rl.deferCounts, D.nTiles, rl.deferCounts + D.nTiles, R.scan, R.stream);
 			D.fse->resolve(&rl, (void*) R.stream);
+		}
 		if (e0) R.span(0, e0, R.mark());
 	}
 	else
@@ -1662,6 +1672,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	D.varyingsSize = V->varyingsSize;
 	D.lateZ = S.lateZ;
 	D.words = H.wordPath;
+	D.fragWords = H.wordPath ? H.nWords : (H.describedSize + 3u) / 4u;
 	D.fragBits = bitsFor((uint64_t) H.width * H.height + 1);
 	D.fse = fse;
 	D.evDraw = evDraw;
@@ -1795,7 +1806,7 @@ struct SRPBatch
 	uint32_t* queues = nullptr;
 	ScanScratch scan;
 	SortScratch sort;
-	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1, 1, 1};
+	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1, 1, 1, 8};
 	int cus = 256;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -1836,6 +1847,34 @@ void batchUpload(SRPBatch* B)
 	HIP_CHECK(hipMemcpyAsync(B->dVsSeg, B->hVsSeg.data(), nd * sizeof(SrpSeg), hipMemcpyHostToDevice, R.stream));
 	HIP_CHECK(hipMemcpyAsync(B->dRasterSeg, B->hRasterSeg.data(), nd * sizeof(SrpSeg), hipMemcpyHostToDevice, R.stream));
 	HIP_CHECK(hipStreamSynchronize(R.stream));
+}
+
+/* Tile side of the batch's draws from its tuning: the draws' tile grids,
+ * their ranges of the batch's tiles and the tile-range buffer (the caller
+ * sizes and uploads the batch afterwards). */
+void batchRetile(SRPBatch* B)
+{
+	const uint32_t side = (B->tuning.tileSize == 16) ? 16u : 8u;
+	uint32_t tileBase = 0;
+	for (size_t d = 0; d < B->cmds.size(); d++)
+	{
+		SrpDrawParams* H = batchParams(B, d);
+		H->tileShift = (side == 16u) ? 4u : 3u;
+		H->tilesX = (H->width + side - 1u) / side;
+		H->tilesY = (H->height + side - 1u) / side;
+		const uint32_t nTiles = H->tilesX * H->tilesY;
+		B->hDraws[d].tileBase = tileBase;
+		B->hRasterSeg[d].first = tileBase;
+		B->hRasterSeg[d].count = nTiles;
+		tileBase += nTiles;
+	}
+	if (tileBase != B->totalTiles)
+	{
+		HIP_CHECK(hipDeviceSynchronize());
+		if (B->tileRange) HIP_CHECK(hipFree(B->tileRange));
+		HIP_CHECK(hipMalloc((void**) &B->tileRange, ((size_t) 2 * tileBase + 2) * sizeof(uint32_t)));
+	}
+	B->totalTiles = tileBase;
 }
 
 AsmArgs batchAsmArgs(SRPBatch* B, bool emit)
@@ -1967,13 +2006,14 @@ void batchLaunch(SRPBatch* B, hipStream_t s)
 	rl.nTiles = B->totalTiles;
 	rl.lateZ = B->lateZ;
 	rl.words = B->words;
-	rl.tile = 8;
-	rl.batched = 1;
+	rl.tile = (B->tuning.tileSize == 16) ? 16u : 8u;
+	rl.batched = (rl.tile == 8u) ? 1u : 0u;   /* 8x8: a wave a tile; 16x16: four (a block) */
 	rl.bigPixel = B->tuning.pixelRuns ? 1u : 0u;
 	rl.depthCull = B->tuning.depthCull ? 1u : 0u;
-	uint32_t waves = (uint32_t) (B->cus * B->tuning.wavesPerCU);
-	if (waves > B->totalTiles) waves = B->totalTiles;
-	rl.nBlocks = waves ? waves : 1u;
+	const uint32_t perTile = (rl.tile == 16u) ? 4u : 1u;
+	uint32_t workers = (uint32_t) (B->cus * B->tuning.wavesPerCU) / perTile;
+	if (workers > B->totalTiles) workers = B->totalTiles;
+	rl.nBlocks = workers ? workers : 1u;
 	rl.seg = B->dRasterSeg;
 	rl.nSeg = (uint32_t) B->cmds.size();
 	rl.xcdQueues = B->tuning.xcdQueues ? 1u : 0u;
@@ -2172,9 +2212,10 @@ SRPBatch* buildBatch(const SRPDrawCommand* cmds, size_t n, const char* caller)
 		H.cacheSize = prep[d].cacheSize;
 		H.clip = B->clip + clipOff * 4;
 		H.varyings = B->varyings + varOff;
-		H.tileShift = 3u;
-		H.tilesX = (H.width + 7u) / 8u;
-		H.tilesY = (H.height + 7u) / 8u;
+		const uint32_t side = (B->tuning.tileSize == 16) ? 16u : 8u;
+		H.tileShift = (side == 16u) ? 4u : 3u;
+		H.tilesX = (H.width + side - 1u) / side;
+		H.tilesY = (H.height + side - 1u) / side;
 		H.maskBox = (uint32_t) B->tuning.maskBox;
 		H.stats = R.dStats;
 		clipOff += prep[d].cacheSize;
@@ -2356,20 +2397,24 @@ void srpBatchSetTuning(SRPBatch* B, const SRPBatchTuning* t)
 	resolvePending();
 	if (!B || !t)
 		return;
-	if (t->wavesPerCU < 1 || t->wavesPerCU > 32 || t->maskBox < 0 || t->maskBox > SRP_SMALL_BBOX)
+	if (t->wavesPerCU < 1 || t->wavesPerCU > 32 || t->maskBox < 0 || t->maskBox > SRP_SMALL_BBOX ||
+	    (t->tileSize != 8 && t->tileSize != 16))
 	{
 		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, "srpBatchSetTuning",
-		        "Waves per CU must lie in 1..32 and the mask box in 0..%d", SRP_SMALL_BBOX);
+		        "Waves per CU must lie in 1..32, the mask box in 0..%d and the tile size be 8 or 16", SRP_SMALL_BBOX);
 		return;
 	}
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
 	batchDropGraph(B);
-	const bool resize = t->maskBox != B->tuning.maskBox;
+	const bool retile = t->tileSize != B->tuning.tileSize;
+	const bool resize = retile || t->maskBox != B->tuning.maskBox;
 	B->tuning = *t;
 	if (!B->cmds.empty())
 	{
+		if (retile)
+			batchRetile(B);
 		if (resize)
-			batchSize(B, "srpBatchSetTuning");   /* the mask box changes the number of tile pairs */
+			batchSize(B, "srpBatchSetTuning");   /* tile size and mask box change the number of tile pairs */
 		else
 			batchUpload(B);
 	}
@@ -2391,38 +2436,46 @@ double srpBatchAutotune(SRPBatch* B, int trials, SRPBatchTuning* best)
 		return 0.0;
 	if (trials < 1) trials = 1;
 	Runtime& R = rt();
+	/* 8x8 tiles, a wave each (the batched kernel): waves per CU x mask box x
+	 * queues x pixel runs; 16x16 tiles, four waves each (a lane a pixel, no
+	 * binning masks or runs): waves per CU x queues */
 	const int waves[] = {4, 8, 16, 28};
 	const int boxes[] = {0, 16, 24};
 	const int queues[] = {0, 1};
 	const int runs[] = {0, 1};
+	std::vector<SRPBatchTuning> cands;
+	for (int box : boxes)
+		for (int w : waves)
+			for (int q : queues)
+				for (int pr : runs)
+					cands.push_back({w, box, q, pr, B->tuning.depthCull, 8});
+	for (int w : waves)
+		for (int q : queues)
+			cands.push_back({w, SRP_SMALL_BBOX, q, 1, B->tuning.depthCull, 16});
 	SRPBatchTuning bestT = B->tuning;
 	double bestMs = 1e30;
 	hipEvent_t a, b;
 	HIP_CHECK(hipEventCreate(&a));
 	HIP_CHECK(hipEventCreate(&b));
-	for (int box : boxes)
-		for (int w : waves)
-			for (int q : queues)
-				for (int pr : runs)
-			{
-				SRPBatchTuning t = {w, box, q, pr, B->tuning.depthCull};
-				srpBatchSetTuning(B, &t);
-				batchFlushPending(B);
-				batchLaunch(B, R.stream);   /* warm-up */
-				HIP_CHECK(hipEventRecord(a, R.stream));
-				for (int k = 0; k < trials; k++)
-					batchLaunch(B, R.stream);
-				HIP_CHECK(hipEventRecord(b, R.stream));
-				HIP_CHECK(hipEventSynchronize(b));
-				float ms = 0.0f;
-				HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-				const double per = (double) ms / trials;
-				if (per < bestMs)
-				{
-					bestMs = per;
-					bestT = t;
-				}
-			}
+	for (const SRPBatchTuning& t : cands)
+	{
+		srpBatchSetTuning(B, &t);
+		batchFlushPending(B);
+		batchLaunch(B, R.stream);   /* warm-up */
+		HIP_CHECK(hipEventRecord(a, R.stream));
+		for (int k = 0; k < trials; k++)
+			batchLaunch(B, R.stream);
+		HIP_CHECK(hipEventRecord(b, R.stream));
+		HIP_CHECK(hipEventSynchronize(b));
+		float ms = 0.0f;
+		HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+		const double per = (double) ms / trials;
+		if (per < bestMs)
+		{
+			bestMs = per;
+			bestT = t;
+		}
+	}
 	HIP_CHECK(hipEventDestroy(a));
 	HIP_CHECK(hipEventDestroy(b));
 	srpBatchSetTuning(B, &bestT);

@@ -21,9 +21,9 @@ class BatchScenes:
         L.srpBenchBatchRead.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         if gpu:
             L.srpBenchBatchCapture.restype = ctypes.c_int
-            L.srpBenchBatchAutotune.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int * 4)]
+            L.srpBenchBatchAutotune.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int * 5)]
             L.srpBenchBatchAutotune.restype = ctypes.c_double
-            L.srpBenchBatchSetTuning.argtypes = [ctypes.c_int] * 4
+            L.srpBenchBatchSetTuning.argtypes = [ctypes.c_int] * 5
             L.srpBenchBatchInfo.argtypes = [ctypes.POINTER(ctypes.c_uint64 * 4)]
         self.n = 0
 
@@ -57,12 +57,14 @@ class BatchScenes:
         return self.lib.srpBenchBatchCapture()
 
     def autotune(self, trials):
-        out = (ctypes.c_int * 4)()
+        """Mean frame ms of the best candidate and it: (waves/CU, mask box,
+        xcd queues, pixel runs, tile size)."""
+        out = (ctypes.c_int * 5)()
         ms = self.lib.srpBenchBatchAutotune(trials, ctypes.byref(out))
         return ms, tuple(out)
 
-    def set_tuning(self, waves, box, xcd, runs=1):
-        self.lib.srpBenchBatchSetTuning(waves, box, xcd, runs)
+    def set_tuning(self, waves, box, xcd, runs=1, tile=8):
+        self.lib.srpBenchBatchSetTuning(waves, box, xcd, runs, tile)
 
     def regenerate(self, k, seed):
         self.lib.srpBenchBatchRegenerate.argtypes = [ctypes.c_int, ctypes.c_uint64]

@@ -281,7 +281,7 @@ void srpBenchBatchRead(int k, uint32_t* color, float* depth, uint8_t* stencil)
 #ifdef SRP_GPU_BUILD
 int srpBenchBatchCapture(void) { return g_bb.batch ? srpBatchCapture(g_bb.batch) : -1; }
 
-double srpBenchBatchAutotune(int trials, int out[4])
+double srpBenchBatchAutotune(int trials, int out[5])
 {
 	SRPBatchTuning t;
 	const double ms = srpBatchAutotune(g_bb.batch, trials, &t);
@@ -289,12 +289,13 @@ double srpBenchBatchAutotune(int trials, int out[4])
 	out[1] = t.maskBox;
 	out[2] = t.xcdQueues;
 	out[3] = t.pixelRuns;
+	out[4] = t.tileSize;
 	return ms;
 }
 
-void srpBenchBatchSetTuning(int waves, int box, int xcd, int runs)
+void srpBenchBatchSetTuning(int waves, int box, int xcd, int runs, int tile)
 {
-	SRPBatchTuning t = {waves, box, xcd, runs, 1};
+	SRPBatchTuning t = {waves, box, xcd, runs, 1, tile};
 	srpBatchSetTuning(g_bb.batch, &t);
 }
 

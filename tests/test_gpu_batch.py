@@ -47,6 +47,31 @@ def test_batch_matches_draw_by_draw_and_oracle(gpu_batch, oracle_batch, n, scale
     assert frames_equal(batched, ref) == []
 
 
+@pytest.mark.parametrize("tile", [8, 16])
+def test_full_scale_batch_autotuned_matches_oracle(gpu_batch, oracle_batch, tile):
+    """The batched-scenes workload at full size (64 scenes, 1.86 M
+    triangles, framebuffers up to 2048 x 2048): autotuned, then at the best
+    waves/CU of each tile size, captured in a graph and replayed, every
+    scene's frame equals the oracle's (about 4 s of CPU)."""
+    print(f"[batch] full scale, tile {tile}", flush=True)
+    ref = oracle_frames(oracle_batch, 64, 1.0, C4_SEED)
+    gb = gpu_batch
+    gb.setup(64, 1.0, C4_SEED, batched=True)
+    try:
+        ms, best = gb.autotune(1)
+        print(f"[batch] autotune {ms:.2f} ms: {best}", flush=True)
+        waves, box, xcd, runs, _ = best
+        gb.set_tuning(waves, box, xcd, runs, tile)
+        gb.frame(True)
+        assert frames_equal(gb.read_all(), ref) == []
+        assert gb.capture() == 0
+        gb.frame(True)
+        gb.frame(True)
+        assert frames_equal(gb.read_all(), ref) == []
+    finally:
+        gb.finish()
+
+
 def test_graph_replays_match_eager(gpu_batch):
     """The captured frame, replayed several times with eager work (draw by
     draw frames) in between, equals the eager batch frame every time."""
@@ -71,18 +96,20 @@ def test_tunings_give_identical_frames(gpu_batch):
     try:
         gb.frame(True)
         base = gb.read_all()
-        for waves, box, xcd, runs in [(1, 0, 0, 0), (2, 8, 1, 1), (16, 16, 0, 0), (32, 24, 1, 1), (4, 0, 1, 1),
-                                      (8, 24, 0, 0)]:
-            gb.set_tuning(waves, box, xcd, runs)
+        for waves, box, xcd, runs, tile in [(1, 0, 0, 0, 8), (2, 8, 1, 1, 8), (16, 16, 0, 0, 8), (32, 24, 1, 1, 8),
+                                            (4, 0, 1, 1, 16), (8, 24, 0, 0, 8), (16, 24, 1, 1, 16),
+                                            (28, 16, 0, 1, 16), (8, 16, 1, 1, 8)]:
+            gb.set_tuning(waves, box, xcd, runs, tile)
             gb.frame(True)
-            assert frames_equal(gb.read_all(), base) == [], (waves, box, xcd, runs)
+            assert frames_equal(gb.read_all(), base) == [], (waves, box, xcd, runs, tile)
     finally:
         gb.finish()
 
 
 def test_set_tuning_rejects_out_of_range(gpu_batch):
-    """srpBatchSetTuning reports values outside waves/CU 1..32 or mask box
-    0..24 through the message callback and keeps the batch as it was."""
+    """srpBatchSetTuning reports values outside waves/CU 1..32, mask box
+    0..24 or tile sizes other than 8 and 16 through the message callback and
+    keeps the batch as it was."""
     import ctypes
     gb = gpu_batch
     gb.setup(8, 0.05, 13, batched=True)
@@ -99,9 +126,9 @@ def test_set_tuning_rejects_out_of_range(gpu_batch):
         gb.frame(True)
         base = gb.read_all()
         gb.lib.srpSetMessageCallback(Callback(cb, None))
-        for waves, box in [(0, 16), (33, 16), (8, 25), (8, -1)]:
-            gb.set_tuning(waves, box, 1, 1)
-        assert got == ["srpBatchSetTuning"] * 4
+        for waves, box, tile in [(0, 16, 8), (33, 16, 8), (8, 25, 8), (8, -1, 8), (8, 16, 12), (8, 16, 32)]:
+            gb.set_tuning(waves, box, 1, 1, tile)
+        assert got == ["srpBatchSetTuning"] * 6
         gb.frame(True)
         assert frames_equal(gb.read_all(), base) == []
     finally:
@@ -115,11 +142,12 @@ def test_autotuner_picks_a_candidate_and_keeps_results(gpu_batch):
     try:
         gb.frame(True)
         base = gb.read_all()
-        ms, (waves, box, xcd, runs) = gb.autotune(2)
+        ms, (waves, box, xcd, runs, tile) = gb.autotune(2)
         print(f"[batch] autotune: {ms:.3f} ms with waves/CU {waves}, mask box {box}, xcd queues {xcd}, "
-              f"pixel runs {runs}", flush=True)
+              f"pixel runs {runs}, tile {tile}", flush=True)
         assert ms > 0
         assert waves in (4, 8, 16, 28) and box in (0, 16, 24) and xcd in (0, 1) and runs in (0, 1)
+        assert tile in (8, 16)
         gb.frame(True)
         assert frames_equal(gb.read_all(), base) == []
     finally:
