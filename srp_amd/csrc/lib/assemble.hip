@@ -1211,6 +1211,28 @@ void launchExpandSpans(const AsmArgs& A, uint32_t blocks, hipStream_t s)
 		                   A.pairVals, A.pairRec, A.pairMask, A.itemCap);
 }
 
+void launchClipped(const AsmArgs& A, bool emit, hipStream_t s, uint32_t clipped, uint32_t clipHint)
+{
+	const uint32_t blocks = A.nBlocks;
+	if (!blocks) return;
+	uint32_t clipBlocks;
+	if (clipped != kClipUnknown)
+	{
+		const uint32_t need = (uint32_t) (((uint64_t) clipped + 7u) / 8u);
+		clipBlocks = need < 2048u ? need : 2048u;
+	}
+	else
+	{
+		const uint64_t want = ((uint64_t) clipHint + clipHint / 4u + 7u) / 8u;
+		clipBlocks = (uint32_t) (want < 256u ? 256u : (want > 2048u ? 2048u : want));
+		clipBlocks = clipBlocks < blocks ? clipBlocks : blocks;
+	}
+	if (!clipBlocks)
+		return;
+	if (emit) hipLaunchKernelGGL(k_assemble_clipped<true>, dim3(clipBlocks), dim3(128), 0, s, A);
+	else hipLaunchKernelGGL(k_assemble_clipped<false>, dim3(clipBlocks), dim3(128), 0, s, A);
+}
+
 void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped, uint32_t clipHint)
 {
 	const uint32_t blocks = A.nBlocks;
@@ -1227,30 +1249,9 @@ void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32
 			 * (the empty grid of a draw that clips nothing still costs its
 			 * dispatch), and never more blocks than the draw's main
 			 * assembly grid. */
-			uint32_t clipBlocks;
-			if (clipped != kClipUnknown)
-			{
-				const uint32_t need = (uint32_t) (((uint64_t) clipped + 7u) / 8u);
-				clipBlocks = need < 2048u ? need : 2048u;
-			}
-			else
-			{
-				const uint64_t want = ((uint64_t) clipHint + clipHint / 4u + 7u) / 8u;
-				clipBlocks = (uint32_t) (want < 256u ? 256u : (want > 2048u ? 2048u : want));
-				clipBlocks = clipBlocks < blocks ? clipBlocks : blocks;
-			}
-			if (emit)
-			{
-				hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);
-				if (clipBlocks)
-					hipLaunchKernelGGL(k_assemble_clipped<true>, dim3(clipBlocks), dim3(128), 0, s, A);
-			}
-			else
-			{
-				hipLaunchKernelGGL(k_assemble_tris<false>, dim3(blocks), dim3(128), 0, s, A);
-				if (clipBlocks)
-					hipLaunchKernelGGL(k_assemble_clipped<false>, dim3(clipBlocks), dim3(128), 0, s, A);
-			}
+			if (emit) hipLaunchKernelGGL(k_assemble_tris<true>, dim3(blocks), dim3(128), 0, s, A);
+			else hipLaunchKernelGGL(k_assemble_tris<false>, dim3(blocks), dim3(128), 0, s, A);
+			launchClipped(A, emit, s, clipped, clipHint);
 			break;
 		}
 		case 1:

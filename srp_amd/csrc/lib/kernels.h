@@ -98,6 +98,9 @@ struct AsmArgs
 constexpr uint32_t kClipUnknown = 0xFFFFFFFFu;
 void launchAssemble(const AsmArgs& A, int kind, bool emit, hipStream_t s, uint32_t clipped = kClipUnknown,
                     uint32_t clipHint = 0);
+/* The clipped-triangle kernel of a triangle launch alone (the same grid
+ * rules), for a caller that runs it on a stream of its own. */
+void launchClipped(const AsmArgs& A, bool emit, hipStream_t s, uint32_t clipped, uint32_t clipHint);
 /* After the emitting pass of filled triangles: write the spans' pairs
  * (`blocks` blocks of 4 waves stride over the device-side span count). */
 void launchExpandSpans(const AsmArgs& A, uint32_t blocks, hipStream_t s);

@@ -144,6 +144,8 @@ struct DrawKeyHash
 struct Runtime
 {
 	hipStream_t stream = nullptr;
+	hipStream_t side = nullptr;       /* a single draw's clipped-triangle emitting kernel */
+	hipEvent_t forkEv = nullptr, joinEv = nullptr;
 	SrpDrawParams* hParams = nullptr;   /* pinned staging */
 	SrpDrawParams* dParams = nullptr;
 	uint8_t* hUniform = nullptr;        /* pinned staging */
@@ -245,6 +247,9 @@ struct Runtime
 	{
 		HIP_CHECK(hipSetDevice(0));
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+		HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+		HIP_CHECK(hipEventCreateWithFlags(&forkEv, hipEventDisableTiming));
+		HIP_CHECK(hipEventCreateWithFlags(&joinEv, hipEventDisableTiming));
 		HIP_CHECK(hipHostMalloc((void**) &hParams, kParamsBlock, hipHostMallocDefault));
 		HIP_CHECK(hipMalloc((void**) &dParams, kParamsBlock));
 		HIP_CHECK(hipHostMalloc((void**) &hTotals, kTotalsBytes + 64, hipHostMallocMapped));
@@ -1277,7 +1282,20 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 	launchFit(totals, caps, speculative ? R.dTotals : nullptr, speculative ? ++R.countsSeq : 0u, R.stream);
 
 	const uint32_t clipped = speculative ? (caps.clipped ? kClipUnknown : 0u) : R.hTotals[4];
-	launchAssemble(A, D.kind, true, R.stream, clipped, D.clipHint);
+	if (D.kind == 0 && clipped != 0u)
+	{
+		/* the clipped triangles' emitting kernel runs beside the others'
+		 * on a second stream (disjoint outputs; its few groups are latency
+		 * bound), joined before the sort */
+		HIP_CHECK(hipEventRecord(R.forkEv, R.stream));
+		HIP_CHECK(hipStreamWaitEvent(R.side, R.forkEv, 0));
+		launchClipped(A, true, R.side, clipped, D.clipHint);
+		HIP_CHECK(hipEventRecord(R.joinEv, R.side));
+		launchAssemble(A, D.kind, true, R.stream, 0u, 0u);
+		HIP_CHECK(hipStreamWaitEvent(R.stream, R.joinEv, 0));
+	}
+	else
+		launchAssemble(A, D.kind, true, R.stream, clipped, D.clipHint);
 	if (D.backend == 0)
 	{
 		/* spans: the grid strides over the device-side count */

@@ -9,7 +9,8 @@ and the GPU path (GPU tests) without the reference itself being present.
 
 The heavy cases are the full-size workloads whose CPU frames take minutes
 (the primitive mix at 1920x1080 covers 3.9e9 fragments); the GPU tests
-compare their frames with these hashes directly.
+compare their frames with these hashes directly.  `--heavy` computes only
+the cases the file does not hold yet (`--redo` recomputes all of them).
 """
 import hashlib
 import json
@@ -42,6 +43,12 @@ def heavy_cases():
         dict(name="cfg/mix", width=1920, height=1080, p0=500000, p1=100000, p2=100000, seed=0x5EED0003),
         dict(name="cfg/terrain", width=1920, height=1080, p2=0),
         dict(name="cfg/terrain", width=1920, height=1080, p2=300),
+    ] + [
+        # the 1080p terrain from more of its camera path: training poses,
+        # the timed pose (256), and test and validation poses of the
+        # learned shader
+        dict(name="cfg/terrain", width=1920, height=1080, p2=p)
+        for p in (37, 64, 128, 192, 256, 280, 319, 333, 351)
     ]
 
 
@@ -63,11 +70,17 @@ def main():
     heavy = "--heavy" in sys.argv
     table = {}
     if heavy:
+        out = "reference_hashes_heavy.json"
+        path = os.path.join(HERE, out)
+        if "--redo" not in sys.argv and os.path.exists(path):
+            with open(path) as f:
+                table = json.load(f)["cases"]
         for c in heavy_cases():
+            if case_id(c) in table:
+                continue
             ref.reset_stats("srpRefResetStats") if hasattr(ref.lib, "srpRefResetStats") else None
             table[case_id(c)] = digest(ref.run(**c))
             print("done", case_id(c), flush=True)
-        out = "reference_hashes_heavy.json"
         meta = {}
     else:
         tex = write_noise_png("/tmp/srp_fixture_noise1024.png", 1024, NOISE_SEED)

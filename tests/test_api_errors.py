@@ -31,6 +31,8 @@ def api():
 
     cb = CALLBACK(on_message)
     lib.srpSetMessageCallback.argtypes = [_Callback]
+    # pointer arguments are passed whole whichever test runs first
+    lib.srpNeuralNetSetEncoding.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.srpSetMessageCallback(_Callback(cb, None))
     yield lib, got
     lib.srpSetMessageCallback(_Callback(CALLBACK(), None))
@@ -57,6 +59,9 @@ def test_encoding_step_out_of_range(api, bad):
     assert "feature 5 is not in (0, 1024)" in got[0][1]
 
 
+_HALF_STEPS = np.full(8, 0.5, np.float32)   # alive for the whole module
+
+
 @pytest.mark.parametrize("call, fn, text", [
     (lambda L: L.srpNeuralDatasetStage(None, None, 0), "srpNeuralDatasetStage", "Invalid dataset, framebuffer or slot (0)"),
     (lambda L: L.srpNeuralDatasetAppend(None, None, ctypes.c_float(1.0), ctypes.c_uint64(1)),
@@ -75,8 +80,7 @@ def test_encoding_step_out_of_range(api, bad):
     (lambda L: L.srpNeuralNetSetParams(None, None), "srpNeuralNetSetParams", "Invalid network or parameter array"),
     (lambda L: L.srpNeuralNetGetEncoding(None, None), "srpNeuralNetGetEncoding", "Invalid network or step array"),
     (lambda L: L.srpNeuralNetSetEncoding(None, None), "srpNeuralNetSetEncoding", "Invalid step array"),
-    (lambda L: L.srpNeuralNetSetEncoding(None, np.full(8, 0.5, np.float32).ctypes.data),
-     "srpNeuralNetSetEncoding", "Invalid network"),
+    (lambda L: L.srpNeuralNetSetEncoding(None, _HALF_STEPS.ctypes.data), "srpNeuralNetSetEncoding", "Invalid network"),
 ])
 def test_missing_objects_are_reported(api, call, fn, text):
     """Every dataset and training call given no object reports it and

@@ -91,6 +91,21 @@ def test_reference_fixture_large(oracle, tex):
         assert digest(oracle.run(**cc)) == HASHES["cases"][case_id(c)]
 
 
+HEAVY = json.load(open(os.path.join(GOLDEN, "reference_hashes_heavy.json")))["cases"]
+TERRAIN_CASES = sorted(k for k in HEAVY if "name=cfg/terrain" in k)
+
+
+@pytest.mark.parametrize("case_id_", TERRAIN_CASES)
+def test_terrain_1080p_fixture(oracle, case_id_):
+    """The full 1080p terrain (1,000,000 triangles) from each camera pose the
+    reference-written full-size fixtures hold."""
+    kw = {}
+    for part in case_id_.split(","):
+        k, v = part.split("=", 1)
+        kw[k] = v if k == "name" else int(v)
+    assert digest(oracle.run(**kw)) == HEAVY[case_id_]
+
+
 def same(a, b):
     return (np.array_equal(a.color, b.color) and np.array_equal(a.depth.view(np.uint32), b.depth.view(np.uint32))
             and np.array_equal(a.stencil, b.stencil))
