@@ -794,6 +794,8 @@ def main():
     ap.add_argument("--lines", type=int, default=100000)
     ap.add_argument("--points", type=int, default=100000)
     ap.add_argument("--tile", type=int, default=0, help="raster tile side (8 or 16); 0 = library default")
+    ap.add_argument("--tile-order", type=int, default=-1,
+                    help="tiles dealt to the XCDs in runs of this many (0: one range per XCD); -1 = library default")
     ap.add_argument("--mask-box", type=int, default=-1,
                     help="largest triangle box side given exact binning masks (0..24); -1 = library default")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle work for the CPU baseline")
@@ -849,6 +851,8 @@ def main():
     if args.mask_box >= 0:
         gpu.srpHipSetRasterOptions.argtypes = [ctypes.c_int] * 3
         gpu.srpHipSetRasterOptions(-1, -1, args.mask_box)
+    if args.tile_order >= 0:
+        gpu.srpHipSetTileOrder(args.tile_order)
     tile = int(gpu.srpHipGetTileSize())
     g, gframe = gpu_frames(wl, tex, args.steps, args.warmup)
     steps = args.steps
@@ -888,7 +892,8 @@ def main():
         "data": {"mesh": "synthetic (seeded UV sphere + noise texture)", "mix": "synthetic (seeded clip-space primitives)",
                  "tris": "synthetic (seeded clip-space triangles)",
                  "terrain": "synthetic (seeded fBm heightfield)"}[args.workload],
-        "config": {"workload": describe(wl), "width": wl["w"], "height": wl["h"], "tile": tile},
+        "config": {"workload": describe(wl), "width": wl["w"], "height": wl["h"], "tile": tile,
+                   "tile_order": int(gpu.srpHipGetTileOrder())},
         "frame_ms": round(ms_per_step, 4),
         "triangles_per_s": round(input_triangles(wl) / (ms_per_step * 1e-3), 1),
         "shaded_per_frame": int(shaded_per_frame),

@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 14u
+#define SRP_ABI_VERSION 15u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -178,6 +178,9 @@ typedef struct SrpRasterLaunch
 	uint32_t words;             /* fast interpolation path usable */
 	uint32_t tile;              /* tile side: 8 or 16 */
 	uint32_t nBlocks;           /* grid size (for the XCD-aware tile order) */
+	uint32_t tileOrder;         /* single draws: 0 = one contiguous range of tiles per
+	                               XCD; n >= 1 = runs of n tiles dealt to the XCDs in
+	                               turn (srpTileOfBlock) */
 	uint32_t batched;           /* batched three-phase kernel (tile 8, one wave per tile) */
 	uint32_t bigPixel;          /* batched: triangles without exact masks are shaded
 	                               pixel-parallel in runs instead of via fragment lists */

@@ -346,14 +346,30 @@ __global__ __launch_bounds__(256) void srp_vs_kernel(SrpVsLaunch L)
 	reinterpret_cast<float4*>(P->clip)[i] = c;
 }
 
-/* Blocks are issued round-robin over the 8 XCDs; give each XCD a
- * contiguous run of tiles so that primitives shared by neighbouring tiles
- * hit the same L2. */
-__device__ __forceinline__ uint32_t xcdOrder(uint32_t b, uint32_t nBlocks)
+/* Blocks are issued round-robin over the 8 XCDs (block b to XCD b % 8).
+ * The tile a block renders:
+ *   run == 0   each XCD gets one contiguous range of tiles, so primitives
+ *              shared by neighbouring tiles hit the same L2; a frame whose
+ *              busy tiles gather in a band of rows then loads a few XCDs;
+ *   run == n   runs of n consecutive tiles are dealt to the XCDs in turn
+ *              (groups of 8n blocks; a last partial group keeps block
+ *              order), so every XCD gets a share of any region of the
+ *              frame and neighbouring tiles within a run share an L2.
+ * Both are permutations of [0, nBlocks). */
+__device__ __forceinline__ uint32_t srpTileOfBlock(uint32_t b, uint32_t nBlocks, uint32_t run)
 {
-	const uint32_t q = nBlocks / 8u, r = nBlocks % 8u;
-	const uint32_t x = b % 8u, k = b / 8u;
-	return x * q + (x < r ? x : r) + k;
+	if (run == 0u)
+	{
+		const uint32_t q = nBlocks / 8u, r = nBlocks % 8u;
+		const uint32_t x = b % 8u, k = b / 8u;
+		return x * q + (x < r ? x : r) + k;
+	}
+	const uint32_t group = 8u * run;
+	const uint32_t m = b / group;
+	if ((m + 1u) * group > nBlocks)
+		return b;
+	const uint32_t r = b - m * group;
+	return m * group + (r % 8u) * run + r / 8u;
 }
 
 /* Triangle rasteriser: each lane owns one pixel of a TILE x TILE tile and
@@ -452,7 +468,7 @@ __global__ __launch_bounds__(256) void srp_raster_kernel(SrpRasterLaunch L)
 	constexpr int TILES_PER_BLOCK = (TILE == 8) ? 4 : 1;
 	const uint32_t t = threadIdx.x;
 	const uint32_t local = (TILE == 8) ? (t & 63u) : t;
-	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks) * TILES_PER_BLOCK + ((TILE == 8) ? (t >> 6) : 0u);
+	const uint32_t tile = srpTileOfBlock(blockIdx.x, L.nBlocks, L.tileOrder) * TILES_PER_BLOCK + ((TILE == 8) ? (t >> 6) : 0u);
 	if (tile >= L.nTiles)
 		return;
 	uint32_t covered = 0, shaded = 0;
@@ -1155,7 +1171,7 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 template <class FS, bool LATE, bool WORDS, int WAVES, bool CULL, bool DEFER = false>
 __global__ __launch_bounds__(64, WAVES) void srp_raster_batched(SrpRasterLaunch L)
 {
-	const uint32_t tile = xcdOrder(blockIdx.x, L.nBlocks);
+	const uint32_t tile = srpTileOfBlock(blockIdx.x, L.nBlocks, L.tileOrder);
 	if (tile >= L.nTiles)
 		return;
 	uint32_t covered = 0, shaded = 0;

@@ -94,6 +94,15 @@ void srpHipSetRasterOptions(int batched, int pixelRuns, int maskBox);
  *  fragments (exact again with culling off). */
 void srpHipSetTileDepthCull(int enable);
 int srpHipGetTileDepthCull(void);
+/** Order in which a single draw's tiles are spread over the GPU's 8 XCDs
+ *  (each has its own L2).  `run` >= 1 deals runs of `run` consecutive
+ *  tiles (row-major) to the XCDs in turn, so a frame whose busy tiles
+ *  cluster in one region still loads every XCD (default 8); 0 gives each
+ *  XCD one contiguous range of tiles.  Values above 64 are reported through the
+ *  message callback and leave the setting unchanged.  The frame is the
+ *  same for every setting. */
+void srpHipSetTileOrder(int run);
+int srpHipGetTileOrder(void);
 /** Deferred shading (on by default) of fragment shaders registered with a
  *  wave entry (SRP_HIP_FRAGMENT_SHADER_WAVE) that do not write depth, in
  *  single triangle draws with the batched rasteriser: the rasteriser does

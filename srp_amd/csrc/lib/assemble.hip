@@ -706,6 +706,178 @@ __device__ __forceinline__ uint32_t drawOfInput(const AsmDraw* D, uint32_t n, ui
  * the limits below only guard numerically degenerate input. */
 constexpr uint32_t CLIP_GROUP = 16;
 
+/* The emitting pass of the filled fan triangles of one clipped triangle
+ * (polygon mode FILL): set up by their fan lanes, binned by the whole
+ * group.  binTriangle scans a box with one lane; here the group's lanes
+ * take work items in turn: each tile of a fan triangle's box (exact-mask
+ * triangles) or the whole span (larger ones).  A tile's mask is found by
+ * the reference's walk restricted to that tile: the row value after
+ * (y - y0) additions of dy (replayed exactly, then added row by row as
+ * triangle.c:101-109 does) and, along the row, (x - x0) additions of dx,
+ * so every bit equals binTriangle's.  Items are numbered fan by fan and,
+ * within a fan, in row-major tile order, and pairs are placed in that
+ * order skipping empty tiles: the same records, pairs and positions as
+ * fan lanes calling fanTriangle one after the other. */
+__device__ __forceinline__ void clipFillFans(const AsmArgs& A, const AsmDraw& W, uint32_t gl, const Vtx& a,
+                                             const Vtx& b, const Vtx& c, bool fan, uint32_t recBase, uint32_t idBase,
+                                             uint32_t itemBase, bool drop)
+{
+	const SrpDrawParams* P = W.P;
+	SrpTri T = {};
+	const bool ok = fan && setupTriangle(P, a, b, c, 0u, T);
+	const bool span = ok && isSpan(P, T);
+	const bool boxed = ok && T.x1 > T.x0 && T.y1 > T.y0;
+	const uint32_t sh = P->tileShift;
+	uint32_t nItem = 0;
+	if (span)
+		nItem = 1u;
+	else if (boxed)
+		nItem = ((((uint32_t) T.x1 - 1u) >> sh) - ((uint32_t) T.x0 >> sh) + 1u) *
+		        ((((uint32_t) T.y1 - 1u) >> sh) - ((uint32_t) T.y0 >> sh) + 1u);
+	/* exclusive scans over the group: work items and records */
+	uint32_t iIncl = nItem, rIncl = ok ? 1u : 0u;
+#pragma unroll
+	for (uint32_t off = 1; off < CLIP_GROUP; off <<= 1)
+	{
+		const uint32_t vi = __shfl_up(iIncl, off, CLIP_GROUP);
+		const uint32_t vr = __shfl_up(rIncl, off, CLIP_GROUP);
+		if (gl >= off)
+		{
+			iIncl += vi;
+			rIncl += vr;
+		}
+	}
+	const uint32_t totI = __shfl(iIncl, (int) CLIP_GROUP - 1, CLIP_GROUP);
+	const uint32_t iEx = iIncl - nItem;
+	const uint32_t r = recBase + rIncl - (ok ? 1u : 0u);
+	bool fits = false;
+	if (ok)
+	{
+		/* a triangle beyond the record capacity (or with a clip vertex
+		 * beyond its capacity) keeps its counted pairs, written as empty
+		 * pairs without a record */
+		fits = r < A.recCap && !drop;
+		T.id = r - idBase;
+		if (fits)
+			A.tris[r] = T;
+		else
+			atomicOr(A.error, 2u);
+	}
+	unsigned long long pairs = 0;   /* pairs of the items before this round */
+	const uint32_t box0 = (uint32_t) T.x0 | ((uint32_t) T.y0 << 16);
+	const uint32_t box1 = (uint32_t) T.x1 | ((uint32_t) T.y1 << 16);
+	for (uint32_t base = 0; base < totI; base += CLIP_GROUP)
+	{
+		const uint32_t item = base + gl;
+		const bool valid = item < totI;
+		/* the fan lane whose items include this one: the first lane whose
+		 * inclusive item count passes it (the counts never decrease) */
+		uint32_t owner = 0;
+#pragma unroll
+		for (uint32_t step = CLIP_GROUP / 2; step > 0; step >>= 1)
+		{
+			const uint32_t v = __shfl(iIncl, (int) (owner + step - 1u), CLIP_GROUP);
+			if (v <= item) owner += step;
+		}
+		const int o = valid ? (int) owner : 0;
+		float l0[3], dx[3], dy[3];
+#pragma unroll
+		for (int e = 0; e < 3; e++)
+		{
+			l0[e] = __shfl(T.l0[e], o, CLIP_GROUP);
+			dx[e] = __shfl(T.dx[e], o, CLIP_GROUP);
+			dy[e] = __shfl(T.dy[e], o, CLIP_GROUP);
+		}
+		const uint32_t ob0 = __shfl(box0, o, CLIP_GROUP), ob1 = __shfl(box1, o, CLIP_GROUP);
+		const uint32_t oflags = __shfl(T.flags, o, CLIP_GROUP);
+		const uint32_t oEx = __shfl(iEx, o, CLIP_GROUP);
+		const bool oSpan = __shfl((int) span, o, CLIP_GROUP) != 0;
+		const uint32_t oRec = __shfl(r, o, CLIP_GROUP);
+		const bool oFits = __shfl((int) fits, o, CLIP_GROUP) != 0;
+		const int x0 = (int) (ob0 & 0xFFFFu), y0 = (int) (ob0 >> 16);
+		const int x1 = (int) (ob1 & 0xFFFFu), y1 = (int) (ob1 >> 16);
+		uint64_t m = 0;
+		uint32_t weight = 0, key = 0;
+		if (valid)
+		{
+			const uint32_t tx0 = (uint32_t) x0 >> sh, ty0 = (uint32_t) y0 >> sh;
+			const uint32_t ntx = (((uint32_t) x1 - 1u) >> sh) - tx0 + 1u;
+			if (oSpan)
+				weight = ntx * ((((uint32_t) y1 - 1u) >> sh) - ty0 + 1u);
+			else
+			{
+				/* exact masks: tileShift 3 */
+				const uint32_t j = item - oEx;
+				const uint32_t ty = ty0 + j / ntx, tx = tx0 + j % ntx;
+				key = W.tileBase + ty * P->tilesX + tx;
+				const int xs = ((int) (tx * 8u) > x0) ? (int) (tx * 8u) : x0;
+				const int xe = ((int) (tx * 8u + 8u) < x1) ? (int) (tx * 8u + 8u) : x1;
+				const int ys = ((int) (ty * 8u) > y0) ? (int) (ty * 8u) : y0;
+				const int ye = ((int) (ty * 8u + 8u) < y1) ? (int) (ty * 8u + 8u) : y1;
+				float row[3];
+#pragma unroll
+				for (int e = 0; e < 3; e++)
+					row[e] = srp::dev::replay(l0[e], dy[e], ys - y0);
+				for (int y = ys; y < ye; y++)
+				{
+					float l[3];
+#pragma unroll
+					for (int e = 0; e < 3; e++)
+						l[e] = srp::dev::replay(row[e], dx[e], xs - x0);
+					for (int x = xs; x < xe; x++)
+					{
+						bool inside = true;
+#pragma unroll
+						for (int e = 0; e < 3; e++)
+						{
+							const bool tl = (oflags >> e) & 1u;
+							inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
+						}
+						if (inside)
+							m |= 1ull << ((((uint32_t) y & 7u) << 3) | ((uint32_t) x & 7u));
+#pragma unroll
+						for (int e = 0; e < 3; e++) l[e] += dx[e];
+					}
+#pragma unroll
+					for (int e = 0; e < 3; e++) row[e] += dy[e];
+				}
+				weight = m ? 1u : 0u;
+			}
+		}
+		/* this round's pairs, in item order */
+		uint32_t wIncl = weight;
+#pragma unroll
+		for (uint32_t off = 1; off < CLIP_GROUP; off <<= 1)
+		{
+			const uint32_t v = __shfl_up(wIncl, off, CLIP_GROUP);
+			if (gl >= off) wIncl += v;
+		}
+		const uint32_t roundPairs = __shfl(wIncl, (int) CLIP_GROUP - 1, CLIP_GROUP);
+		if (weight)
+		{
+			const unsigned long long at = (unsigned long long) itemBase + pairs + (wIncl - weight);
+			if (oSpan)
+			{
+				SrpTri S = {};
+				S.x0 = (uint16_t) x0; S.y0 = (uint16_t) y0;
+				S.x1 = (uint16_t) x1; S.y1 = (uint16_t) y1;
+				appendSpan(A, P, S, W.tileBase, (uint32_t) at, oFits ? oRec : 0xFFFFFFFFu);
+			}
+			else if (at < A.itemCap)
+			{
+				const uint32_t q = (uint32_t) at;
+				A.pairKeys[q] = key;
+				A.pairVals[q] = q;
+				A.pairRec[q] = oFits ? (oRec | 0x80000000u) : 0xFFFFFFFFu;
+				A.pairMask[q] = oFits ? m : 0ull;
+			}
+			if (at + weight > A.itemCap)
+				atomicOr(A.error, 2u);
+		}
+		pairs += roundPairs;
+	}
+}
+
 /* One triangle's Sutherland-Hodgman clip (clipping.c:68-119, :199-258) by
  * a group of CLIP_GROUP lanes, lane j holding polygon vertex j, so nothing
  * lives in per-lane scratch memory.  Each of the six passes runs as the
@@ -948,6 +1120,15 @@ __device__ __forceinline__ void clipGroup(const AsmArgs& A, uint32_t g, bool has
 	a.vref = __shfl(v.vref, 0, CLIP_GROUP);
 	cc.vref = __shfl(v.vref, (int) nextL, CLIP_GROUP);
 	const bool fan = gl >= 1u && (int) gl + 1 < np;
+	if (EMIT && P->st.polygonMode == SRP_POLYGON_MODE_FILL)
+	{
+		/* (the counting pass keeps one lane per fan triangle: its scan
+		 * stops at a tile's first covered pixel, which for the small
+		 * triangles clipping mostly makes costs less than the group's
+		 * shuffles) */
+		clipFillFans(A, W, gl, a, v, cc, fan, recBase, idBase, itemBase, drop);
+		return;
+	}
 	uint32_t myRec = 0, myItems = 0;
 	if (fan)
 		fanTriangle<false, !EMIT>(A, W, a, v, cc, 0u, 0u, 0u, myRec, myItems, false);

@@ -159,6 +159,7 @@ struct Runtime
 	uint32_t maskBox = SRP_SMALL_BBOX;  /* exact binning masks up to this box side */
 	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
 	uint32_t depthCull = 1; /* tile-level depth culling in runs */
+	uint32_t tileOrder = 8; /* single draws: runs of this many tiles dealt to the XCDs in turn (0: ranges) */
 	uint32_t deferShading = 1;  /* wave shaders shade each pixel's visible fragment only */
 	uint32_t lastClipped = 0;   /* triangles the previous draw clipped (grid hint of a new draw) */
 
@@ -1113,6 +1114,18 @@ void srpHipSetDeferredShading(int enable)
 	rt().deferShading = enable ? 1u : 0u;
 }
 int srpHipGetDeferredShading(void) { return (int) rt().deferShading; }
+
+void srpHipSetTileOrder(int run)
+{
+	resolvePending();
+	if (run < 0 || run > 64)
+	{
+		message(SRP_MESSAGE_ERROR, SRP_MESSAGE_SEVERITY_HIGH, __func__, "Tile run %d is not in [0, 64]", run);
+		return;
+	}
+	rt().tileOrder = (uint32_t) run;
+}
+int srpHipGetTileOrder(void) { return (int) rt().tileOrder; }
 int srpHipGetTileDepthCull(void) { return (int) rt().depthCull; }
 
 }  // extern "C"
@@ -1729,6 +1742,7 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	D.rl.depthCull = (R.depthCull && S.depthTest && monotone && !S.stencilEnabled && !S.lateZ) ? 1u : 0u;
 	const uint32_t perBlock = D.rl.batched ? 1u : ((R.tile == 8) ? 4u : 1u);
 	D.rl.nBlocks = (D.nTiles + perBlock - 1) / perBlock;
+	D.rl.tileOrder = R.tileOrder;
 	if (D.emitted)
 	{
 		typedef unsigned long long u64;

@@ -91,6 +91,14 @@ def test_missing_objects_are_reported(api, call, fn, text):
     assert got == [(fn, text)]
 
 
+@pytest.mark.parametrize("run", [-1, 65])
+def test_tile_order_out_of_range(api, run):
+    lib, got = api
+    got.clear()
+    lib.srpHipSetTileOrder(run)
+    assert got == [("srpHipSetTileOrder", f"Tile run {run} is not in [0, 64]")]
+
+
 @pytest.mark.parametrize("fn, args", [
     ("srpFramebufferSetDeferred", (None, True)),
     ("srpFramebufferDownload", (None,)),

@@ -130,6 +130,25 @@ def test_raster_variants(oracle, gpu, batched, runs, box, cull):
         lib.srpHipSetTileDepthCull(1)
 
 
+@pytest.mark.parametrize("run", [0, 1, 7])
+def test_tile_orders(oracle, gpu, noise_tex, run):
+    """The frame does not depend on how a draw's tiles are spread over the
+    XCDs (the default, 8, runs in every other test)."""
+    lib = gpu.lib
+    lib.srpHipSetTileOrder(run)
+    try:
+        assert lib.srpHipGetTileOrder() == run
+        cases = [dict(name="cfg/mesh", width=1920, height=1080, p0=256, p1=512, tex_path=noise_tex),
+                 dict(name="cfg/terrain", width=480, height=270, p0=200, p1=100, p2=7),
+                 dict(name="cfg/tris", p0=2000, seed=0x5EED0001)]
+        cases += [dict(name="cfg/fuzz", seed=s) for s in range(1, 40, 5)]
+        for c in cases:
+            a, b, sa, sb = run_both(oracle, gpu, **c)
+            assert_same(a, b, sa, sb, f"{c} tile order {run}")
+    finally:
+        lib.srpHipSetTileOrder(8)
+
+
 def test_cfg_mesh_small(oracle, gpu, noise_tex):
     a, b, sa, sb = run_both(oracle, gpu, "cfg/mesh", width=480, height=270, p0=32, p1=64, tex_path=noise_tex)
     assert_same(a, b, sa, sb, "cfg/mesh small")
