@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 16u
+#define SRP_ABI_VERSION 15u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -168,13 +168,11 @@ typedef struct SrpRasterLaunch
 {
 	const SrpDrawParams* params;
 	const SrpTri* tris;
+	const uint32_t* tileList;   /* pair indices, grouped by tile, in primitive order */
 	const uint32_t* tileStart;  /* first list entry of each tile */
 	const uint32_t* tileEnd;    /* one past its last entry (start == end: empty) */
-	/* The tiles' lists, entries grouped by tile and in primitive order
-	 * within a tile: */
-	const uint32_t* listRec;    /* triangle record | 0x80000000 if listMask is exact;
-	                               0xFFFFFFFF: no record (a capacity was exceeded) */
-	const unsigned long long* listMask;  /* exact coverage of the tile's 8x8 pixels */
+	const uint32_t* pairRec;    /* triangle record of a pair | 0x80000000 if pairMask is exact */
+	const unsigned long long* pairMask;  /* exact coverage of the tile's 8x8 pixels */
 	uint32_t nTiles;
 	uint32_t lateZ;
 	uint32_t words;             /* fast interpolation path usable */

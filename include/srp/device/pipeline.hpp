@@ -405,7 +405,7 @@ __device__ __forceinline__ void rasterTileSimple(const SrpRasterLaunch& L, const
 
 	for (uint32_t i = begin; i < end; i++)
 	{
-		const uint32_t pr = L.listRec[i];
+		const uint32_t pr = L.pairRec[L.tileList[i]];
 		if (pr == 0xFFFFFFFFu)   /* a pair without a record (a batch's capacity exceeded) covers nothing */
 			continue;
 		const SrpTri* T = &L.tris[pr & 0x7FFFFFFFu];
@@ -732,10 +732,10 @@ __device__ __forceinline__ void rasterTileBatched(const SrpRasterLaunch& L, cons
 		const bool fresh = mine >= filled && mine < avail;
 		if (fresh)
 		{
-			const uint32_t pr = L.listRec[i + mine];
-			const unsigned long long lm = L.listMask[i + mine];   /* loaded beside the record word */
+			const uint32_t pi = L.tileList[i + mine];
+			const uint32_t pr = L.pairRec[pi];
 			sknown = (pr >> 31) != 0;
-			smask = sknown ? lm : 0ull;
+			smask = sknown ? L.pairMask[pi] : 0ull;
 			if (pr != 0xFFFFFFFFu)   /* a pair without a record (capacity exceeded) covers nothing */
 				recIdx[lane] = pr & 0x7FFFFFFFu;
 		}

@@ -136,21 +136,12 @@ struct SortScratch
 	size_t cap = 0, histCap = 0;
 	ScanScratch scan;
 };
-/* With a gather, the last pass writes rec[v] and mask[v] of each item's
- * value v (in sorted order) to the value output and maskOut instead of v. */
-struct SortGather
-{
-	const uint32_t* rec;
-	const unsigned long long* mask;
-	unsigned long long* maskOut;
-};
 /* Stable LSD radix sort of (keys, vals) on the low `bits` bits of the keys.
  * The sorted arrays are returned through *outKeys / *outVals (either the
  * inputs or the scratch buffers).  With nDev, n is a capacity and the count
  * is min(n, *nDev), read on the device (no host synchronisation). */
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& scratch,
-               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr,
-               const SortGather* gather = nullptr);
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr);
 void radixSortReserve(SortScratch& scratch, size_t n);
 
 /* Capacities an emitting pass was sized for before its counts were known,

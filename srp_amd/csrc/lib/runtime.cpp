@@ -188,7 +188,7 @@ struct Runtime
 	std::unordered_map<DrawKey, FitCaps, DrawKeyHash> drawCounts;
 
 	DevBuf uniform, clip, varyings, genVaryings, counts;
-	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, listMask, frags, fragKeys, order, tileCount, spans;
+	DevBuf tris, lines, points, pairKeys, pairVals, pairRec, pairMask, frags, fragKeys, order, tileCount, spans;
 	DevBuf deferList;   /* (pixel, record) of each pixel a deferred-shading draw writes */
 	DevBuf deferCounts; /* pixels each tile lists there, then their offsets */
 	DevBuf deferFrags;  /* those pixels' fragments, rebuilt for shading */
@@ -1264,7 +1264,6 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 	A.genVaryings = genBuf;
 	uint32_t* tileStart = D.tileStart;   /* zeroed by the counting pass */
 	uint32_t* tileEnd = tileStart ? tileStart + D.nTiles : nullptr;
-	unsigned long long* listMask = nullptr;   /* the pairs' masks in list order */
 	if (D.backend == 0)
 	{
 		A.tris = (SrpTri*) take(R.tris, want.records, sizeof(SrpTri), 0, caps.records);
@@ -1272,7 +1271,6 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		A.pairVals = (uint32_t*) take(R.pairVals, want.items, 4, 16, caps.items);
 		A.pairRec = (uint32_t*) take(R.pairRec, want.items, 4, 16, caps.items);
 		A.pairMask = (unsigned long long*) take(R.pairMask, want.items, 8, 16, caps.items);
-		listMask = (unsigned long long*) take(R.listMask, want.items, 8, 16, caps.items);
 		A.spans = (SrpSpan*) take(R.spans, want.spans, sizeof(SrpSpan), 64, caps.spans);
 	}
 	else
@@ -1315,18 +1313,16 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 	{
 		/* spans: the grid strides over the device-side count */
 		launchExpandSpans(A, (uint32_t) std::min<unsigned long long>((caps.spans + 3) / 4, 2048ull), R.stream);
-		/* sorted by tile, the last pass gathering each pair's record word
-		 * and mask into list order */
 		uint32_t *keys, *vals;
-		const SortGather ga = {A.pairRec, A.pairMask, listMask};
-		radixSort(A.pairKeys, A.pairVals, caps.items, bitsFor(D.nTiles), R.sort, &keys, &vals, R.stream, items, &ga);
+		radixSort(A.pairKeys, A.pairVals, caps.items, bitsFor(D.nTiles), R.sort, &keys, &vals, R.stream, items);
 		tileBounds(keys, caps.items, tileStart, tileEnd, R.stream, items);
 		SrpRasterLaunch rl = D.rl;
 		rl.tris = A.tris;
+		rl.tileList = vals;
 		rl.tileStart = tileStart;
 		rl.tileEnd = tileEnd;
-		rl.listRec = vals;
-		rl.listMask = listMask;
+		rl.pairRec = A.pairRec;
+		rl.pairMask = A.pairMask;
 		if (D.fse->resolve && !D.lateZ && rl.batched && R.deferShading)
 		{
 			/* deferred shading: a list entry per pixel at most; the counter
@@ -1834,7 +1830,6 @@ struct SRPBatch
 	SrpSpan* spans = nullptr;                /* recCap entries */
 	uint32_t *pairKeys = nullptr, *pairVals = nullptr, *pairRec = nullptr;
 	unsigned long long* pairMask = nullptr;
-	unsigned long long* listMask = nullptr;  /* the pairs' masks in list order (the sort's gather) */
 	uint8_t* genVaryings = nullptr;
 	uint32_t recCap = 0, itemCap = 0, genCap = 0;
 	uint32_t nRec = 0, nItems = 0, nGen = 0;
@@ -1993,8 +1988,7 @@ bool batchSize(SRPBatch* B, const char* caller)
 	if (recCap > B->recCap || genCap > B->genCap || itemCap > B->itemCap)
 	{
 		HIP_CHECK(hipDeviceSynchronize());
-		void* bufs[] = {B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->listMask,
-		                B->genVaryings};
+		void* bufs[] = {B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings};
 		for (void* p : bufs)
 			if (p) HIP_CHECK(hipFree(p));
 		/* headroom for frames whose data grow; never past the per-call limits */
@@ -2008,7 +2002,6 @@ bool batchSize(SRPBatch* B, const char* caller)
 		HIP_CHECK(hipMalloc((void**) &B->pairVals, (size_t) B->itemCap * 4));
 		HIP_CHECK(hipMalloc((void**) &B->pairRec, (size_t) B->itemCap * 4));
 		HIP_CHECK(hipMalloc((void**) &B->pairMask, (size_t) B->itemCap * 8));
-		HIP_CHECK(hipMalloc((void**) &B->listMask, (size_t) B->itemCap * 8));
 		HIP_CHECK(hipMalloc((void**) &B->genVaryings, (size_t) B->genCap * B->sp->vs->varyingsSize + 512));
 		batchUpload(B);
 	}
@@ -2030,19 +2023,18 @@ void batchLaunch(SRPBatch* B, hipStream_t s)
 	uint32_t* tileEnd = B->tileRange + B->totalTiles;
 	fillU32(B->tileRange, 0u, (size_t) 2 * B->totalTiles, s);
 	uint32_t *keys, *vals;
-	const SortGather ga = {B->pairRec, B->pairMask, B->listMask};
-	radixSort(B->pairKeys, B->pairVals, B->itemCap, bitsFor(B->totalTiles), B->sort, &keys, &vals, s, B->totals + 2,
-	          &ga);
+	radixSort(B->pairKeys, B->pairVals, B->itemCap, bitsFor(B->totalTiles), B->sort, &keys, &vals, s, B->totals + 2);
 	tileBounds(keys, B->itemCap, tileStart, tileEnd, s, B->totals + 2);
 	fillU32(B->queues, 0u, 8, s);
 	SrpRasterLaunch rl;
 	memset(&rl, 0, sizeof(rl));
 	rl.params = nullptr;
 	rl.tris = B->tris;
+	rl.tileList = vals;
 	rl.tileStart = tileStart;
 	rl.tileEnd = tileEnd;
-	rl.listRec = vals;
-	rl.listMask = B->listMask;
+	rl.pairRec = B->pairRec;
+	rl.pairMask = B->pairMask;
 	rl.nTiles = B->totalTiles;
 	rl.lateZ = B->lateZ;
 	rl.words = B->words;
@@ -2360,8 +2352,7 @@ void srpFreeBatch(SRPBatch* B)
 	HIP_CHECK(hipStreamSynchronize(rt().stream));
 	batchDropGraph(B);
 	void* bufs[] = {B->dBlocks, B->dDraws, B->dVsSeg, B->dRasterSeg, B->clip, B->varyings, B->counts, B->totals,
-	                B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->listMask, B->genVaryings,
-	                B->tileRange,
+	                B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings, B->tileRange,
 	                B->queues, B->scan.buf, B->scan.buf64, B->sort.keys, B->sort.vals, B->sort.hist,
 	                B->sort.scan.buf, B->sort.scan.buf64};
 	for (void* p : bufs)

@@ -301,16 +301,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys,
  * of 256 items ranks its items per digit with wave ballots, then by wave.
  * FUSED: hist holds raw block-major counts (hist[block * 256 + digit]) and
  * the block finds its digits' offsets itself; otherwise hist holds the
- * exclusive scan of the digit-major counts.  GATHER (a last pass): an
- * item's value v is not written; gatherRec[v] and gatherMask[v] are, to
- * valsOut and maskOut. */
-template <bool FUSED, bool GATHER>
+ * exclusive scan of the digit-major counts. */
+template <bool FUSED>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
                                                               uint32_t* keysOut, uint32_t* valsOut, size_t n,
                                                               const uint32_t* nDev, uint32_t shift, const uint32_t* hist,
-                                                              uint32_t nBlocks, const uint32_t* gatherRec,
-                                                              const unsigned long long* gatherMask,
-                                                              unsigned long long* maskOut)
+                                                              uint32_t nBlocks)
 {
 	if (nDev && *nDev < n) n = *nDev;
 	/* blocks past the device count (a capacity-sized grid) have nothing to
@@ -384,25 +380,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 		for (int k = 0; k < RS_SLICES * 4; k++) sliceCnt[k][t] = 0;
 		__syncthreads();
 		uint32_t key[RS_SLICES], val[RS_SLICES], d[RS_SLICES], rank[RS_SLICES];
-		unsigned long long msk[RS_SLICES];
 		bool valid[RS_SLICES];
 #pragma unroll
 		for (int sl = 0; sl < RS_SLICES; sl++)
 		{
 			const size_t i = base + (size_t) sl * RS_THREADS + t;
 			valid[sl] = i < end;
-			key[sl] = 0; val[sl] = 0; d[sl] = 0; msk[sl] = 0;
+			key[sl] = 0; val[sl] = 0; d[sl] = 0;
 			if (valid[sl])
 			{
 				key[sl] = keysIn[i];
 				val[sl] = valsIn[i];
 				d[sl] = (key[sl] >> shift) & 255u;
-				if (GATHER)
-				{
-					/* the gathered words are loaded while the items are ranked */
-					msk[sl] = gatherMask[val[sl]];
-					val[sl] = gatherRec[val[sl]];
-				}
 			}
 		}
 #pragma unroll
@@ -439,8 +428,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 				const uint32_t dst = running[d[sl]] + sliceCnt[sl * 4 + wave][d[sl]] + rank[sl];
 				keysOut[dst] = key[sl];
 				valsOut[dst] = val[sl];
-				if (GATHER)
-					maskOut[dst] = msk[sl];
 			}
 		__syncthreads();
 		running[t] += tileTotal[t];
@@ -501,16 +488,14 @@ void radixSortReserve(SortScratch& sc, size_t n)
 }
 
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& sc,
-               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev, const SortGather* gather)
+               uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev)
 {
 	*outKeys = keys;
 	*outVals = vals;
-	if (n == 0 || (!gather && (n <= 1 || bits <= 0)))
+	if (n <= 1 || bits <= 0)
 		return;
-	if (bits <= 0)
-		bits = 8;   /* the gather happens in a pass, so there is one */
 	const uint32_t nBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
-	radixSortReserve(sc, n < 2 ? 2 : n);
+	radixSortReserve(sc, n);
 	const size_t histN = (size_t) 256 * nBlocks;
 	uint32_t* kIn = keys;
 	uint32_t* vIn = vals;
@@ -519,32 +504,20 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 	const bool fused = nBlocks <= RS_FUSED_MAX_BLOCKS;
 	for (int shift = 0; shift < bits; shift += 8)
 	{
-		const bool last = shift + 8 >= bits;
-		const uint32_t* gRec = gather ? gather->rec : nullptr;
-		const unsigned long long* gMask = gather ? gather->mask : nullptr;
-		unsigned long long* mOut = gather ? gather->maskOut : nullptr;
 		if (fused)
 		{
 			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
 			                   sc.hist, 1u, 256u);
-			if (gather && last)
-				hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut,
-				                   vOut, n, nDev, (uint32_t) shift, sc.hist, nBlocks, gRec, gMask, mOut);
-			else
-				hipLaunchKernelGGL((k_radix_scatter<true, false>), dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut,
-				                   vOut, n, nDev, (uint32_t) shift, sc.hist, nBlocks, nullptr, nullptr, nullptr);
+			hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks);
 		}
 		else
 		{
 			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
 			                   sc.hist, nBlocks, 1u);
 			scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
-			if (gather && last)
-				hipLaunchKernelGGL((k_radix_scatter<false, true>), dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut,
-				                   vOut, n, nDev, (uint32_t) shift, sc.hist, nBlocks, gRec, gMask, mOut);
-			else
-				hipLaunchKernelGGL((k_radix_scatter<false, false>), dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut,
-				                   vOut, n, nDev, (uint32_t) shift, sc.hist, nBlocks, nullptr, nullptr, nullptr);
+			hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks);
 		}
 		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
 		uint32_t* tv = vIn; vIn = vOut; vOut = tv;
