@@ -24,6 +24,12 @@
  * For every float x, |x| <= 1e-9 holds exactly when |x| <= this, so the
  * test needs no double arithmetic. */
 #define SRP_EPSILON_F 0x1.12e0bep-30f
+/* The reference's edge test (raster/triangle.c:78-110) admits a pixel on
+ * edge e when l > 0, or when |l| <= EPSILON and the edge is top-left.  For
+ * a top-left edge that is l >= -SRP_EPSILON_F, i.e. l above the next float
+ * below it, this; for any other edge l > 0.  So each edge is one compare
+ * with a per-edge threshold (NaN fails both forms). */
+#define SRP_EDGE_TL_THRESHOLD (-0x1.12e0c0p-30f)
 #define SRP_STAT_SLOTS 64
 #define SRP_STAT_STRIDE 16
 
@@ -162,6 +168,10 @@ typedef struct SrpVsLaunch
 	const SrpSeg* seg;
 	uint32_t nSeg;
 	uint32_t nBlocks;
+	/* when set, block 0 first clears zero[0..zeroCount) (zeroCount <= 256):
+	 * the draw's counters, when its parameters were not uploaded again */
+	uint32_t* zero;
+	uint32_t zeroCount;
 } SrpVsLaunch;
 
 typedef struct SrpRasterLaunch

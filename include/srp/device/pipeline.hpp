@@ -30,6 +30,26 @@ namespace dev {
 /* reference math/utils.h:41-45 (comparison in double precision) */
 __device__ __forceinline__ bool roughlyZero(float x) { return fabsf(x) <= SRP_EPSILON_F; }
 
+/* Per-edge thresholds of the inside test: l[e] > thr[e] for every edge
+ * is (l[e] > 0) || (roughlyZero(l[e]) && topLeft(e)) (SRP_EDGE_TL_THRESHOLD). */
+__device__ __forceinline__ void edgeThresholds(uint32_t flags, float thr[3])
+{
+#pragma unroll
+	for (int e = 0; e < 3; e++)
+		thr[e] = ((flags >> e) & 1u) ? SRP_EDGE_TL_THRESHOLD : 0.0f;
+}
+__device__ __forceinline__ bool insideEdges(const float* l, const float* thr)
+{
+	return (l[0] > thr[0]) & (l[1] > thr[1]) & (l[2] > thr[2]);
+}
+/* The reference's coverage test of a pixel with barycentrics l. */
+__device__ __forceinline__ bool insideTri(const SrpTri& T, const float* l)
+{
+	float thr[3];
+	edgeThresholds(T.flags, thr);
+	return insideEdges(l, thr);
+}
+
 /* reference raster/fragment.c:156-178, :225-245 */
 template <typename T>
 __device__ __forceinline__ bool compare(uint32_t op, T lhs, T rhs)
@@ -312,6 +332,8 @@ __device__ __forceinline__ void flushStats(const SrpDrawParams* P, uint32_t cove
 template <class VS>
 __global__ __launch_bounds__(256) void srp_vs_kernel(SrpVsLaunch L)
 {
+	if (L.zero && blockIdx.x == 0 && threadIdx.x < L.zeroCount)
+		L.zero[threadIdx.x] = 0u;
 	const SrpDrawParams* P = L.params;
 	uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	uint32_t count = L.count;
@@ -413,15 +435,10 @@ __device__ __forceinline__ void rasterTileSimple(const SrpRasterLaunch& L, const
 			continue;
 		const int ny = py - (int) T->y0, nx = px - (int) T->x0;
 		float l[3];
-		bool inside = true;
 #pragma unroll
 		for (int e = 0; e < 3; e++)
-		{
 			l[e] = replay(replay(T->l0[e], T->dy[e], ny), T->dx[e], nx);
-			const bool tl = (T->flags >> e) & 1u;
-			inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
-		}
-		if (!inside)
+		if (!insideTri(*T, l))
 			continue;
 		const float iw[3] = {T->invW[0], T->invW[1], T->invW[2]};
 		const float rec = 1.0f / (iw[0] * l[0] + iw[1] * l[1] + iw[2] * l[2]);
@@ -598,18 +615,6 @@ __device__ __forceinline__ void baryAt(const SrpTri& T, int x, int y, float* l)
 #pragma unroll
 	for (int e = 0; e < 3; e++)
 		l[e] = replay(replay(T.l0[e], T.dy[e], ny), T.dx[e], nx);
-}
-
-__device__ __forceinline__ bool insideTri(const SrpTri& T, const float* l)
-{
-	bool in = true;
-#pragma unroll
-	for (int e = 0; e < 3; e++)
-	{
-		const bool tl = (T.flags >> e) & 1u;
-		in = in && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
-	}
-	return in;
 }
 
 /* Row bases of up to 8 triangles of a tile (the set bits of `round`, in

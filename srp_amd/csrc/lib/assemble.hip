@@ -275,8 +275,11 @@ __device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const Sr
 		 * band keep their masks in registers.  The walk itself is the
 		 * reference's row-major scan of the whole box. */
 		float row[3] = {T.l0[0], T.l0[1], T.l0[2]};
+		float thr[3];
+		srp::dev::edgeThresholds(T.flags, thr);
 		int y = T.y0;
-		const uint32_t allCols = (1u << (tx1 - tx0 + 1u)) - 1u;
+		const uint32_t nCols = tx1 - tx0 + 1u;
+		const uint32_t allCols = (1u << nCols) - 1u;
 		for (uint32_t ty = ty0; ty <= ty1; ty++)
 		{
 			uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
@@ -287,24 +290,24 @@ __device__ __forceinline__ uint32_t binTriangle(const SrpDrawParams* P, const Sr
 				                       (m3 != 0ull ? 8u : 0u);
 				const int xEnd = (!EMIT && found == allCols) ? (int) T.x0 : (int) T.x1;
 				float l[3] = {row[0], row[1], row[2]};
-				for (int x = T.x0; x < xEnd; x++)
+				const uint32_t rowShift = ((uint32_t) y & 7u) << 3;
+				int x = T.x0;
+				/* the row tile by tile: a byte of bits per tile, placed in
+				 * that tile's mask once the tile's part of the row is done */
+				for (uint32_t c = 0; c < nCols && x < xEnd; c++)
 				{
-					bool inside = true;
-					for (int e = 0; e < 3; e++)
+					const int segEnd = ((int) ((tx0 + c + 1u) << 3) < xEnd) ? (int) ((tx0 + c + 1u) << 3) : xEnd;
+					uint32_t bits = 0;
+					for (; x < segEnd; x++)
 					{
-						const bool tl = (T.flags >> e) & 1u;
-						inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
+						bits |= (srp::dev::insideEdges(l, thr) ? 1u : 0u) << ((uint32_t) x & 7u);
+						for (int e = 0; e < 3; e++) l[e] += T.dx[e];
 					}
-					if (inside)
-					{
-						const uint32_t c = ((uint32_t) x >> 3) - tx0;
-						const uint64_t bit = 1ull << ((((uint32_t) y & 7u) << 3) | ((uint32_t) x & 7u));
-						m0 |= c == 0 ? bit : 0ull;
-						m1 |= c == 1 ? bit : 0ull;
-						m2 |= c == 2 ? bit : 0ull;
-						m3 |= c == 3 ? bit : 0ull;
-					}
-					for (int e = 0; e < 3; e++) l[e] += T.dx[e];
+					const uint64_t add = (uint64_t) bits << rowShift;
+					m0 |= c == 0 ? add : 0ull;
+					m1 |= c == 1 ? add : 0ull;
+					m2 |= c == 2 ? add : 0ull;
+					m3 |= c == 3 ? add : 0ull;
 				}
 				for (int e = 0; e < 3; e++) row[e] += T.dy[e];
 			}
@@ -814,7 +817,8 @@ __device__ __forceinline__ void clipFillFans(const AsmArgs& A, const AsmDraw& W,
 				const int xe = ((int) (tx * 8u + 8u) < x1) ? (int) (tx * 8u + 8u) : x1;
 				const int ys = ((int) (ty * 8u) > y0) ? (int) (ty * 8u) : y0;
 				const int ye = ((int) (ty * 8u + 8u) < y1) ? (int) (ty * 8u + 8u) : y1;
-				float row[3];
+				float row[3], thr[3];
+				srp::dev::edgeThresholds(oflags, thr);
 #pragma unroll
 				for (int e = 0; e < 3; e++)
 					row[e] = srp::dev::replay(l0[e], dy[e], ys - y0);
@@ -824,20 +828,14 @@ __device__ __forceinline__ void clipFillFans(const AsmArgs& A, const AsmDraw& W,
 #pragma unroll
 					for (int e = 0; e < 3; e++)
 						l[e] = srp::dev::replay(row[e], dx[e], xs - x0);
+					uint32_t bits = 0;
 					for (int x = xs; x < xe; x++)
 					{
-						bool inside = true;
-#pragma unroll
-						for (int e = 0; e < 3; e++)
-						{
-							const bool tl = (oflags >> e) & 1u;
-							inside = inside && ((l[e] > 0.0f) || (roughlyZero(l[e]) && tl));
-						}
-						if (inside)
-							m |= 1ull << ((((uint32_t) y & 7u) << 3) | ((uint32_t) x & 7u));
+						bits |= (srp::dev::insideEdges(l, thr) ? 1u : 0u) << ((uint32_t) x & 7u);
 #pragma unroll
 						for (int e = 0; e < 3; e++) l[e] += dx[e];
 					}
+					m |= (uint64_t) bits << (((uint32_t) y & 7u) << 3);
 #pragma unroll
 					for (int e = 0; e < 3; e++) row[e] += dy[e];
 				}

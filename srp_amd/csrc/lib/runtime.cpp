@@ -212,6 +212,11 @@ struct Runtime
 	 * published (they follow its uploads on the stream); a parameter patch
 	 * after that (a larger clip-vertex buffer) is waited for with this. */
 	hipEvent_t paramsCopied = nullptr;
+	/* What the device parameter block holds outside its counters (bytes
+	 * [0, kTotalsOff) and the inline uniform from kParamsHead), so that a
+	 * draw with the same parameters need not send them again; size 0: not
+	 * known. */
+	std::vector<uint8_t> sentParams;
 	bool paramsPatched = false;
 
 	hipEvent_t mark()
@@ -1258,6 +1263,7 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 		R.hParams->genVaryings = genBuf;
 		HIP_CHECK(hipMemcpyAsync(&R.dParams->genVaryings, &R.hParams->genVaryings, sizeof(genBuf),
 		                         hipMemcpyHostToDevice, R.stream));
+		R.sentParams.clear();
 		HIP_CHECK(hipEventRecord(R.paramsCopied, R.stream));
 		R.paramsPatched = true;
 	}
@@ -1669,18 +1675,37 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 		AD.tileBase = 0;
 		AD.firstBlock = 0;
 	}
-	HIP_CHECK(hipMemcpyAsync(R.dParams, &H, paramsBytes, hipMemcpyHostToDevice, R.stream));
+	/* The parameters go to the device unless it holds the same already (a
+	 * draw repeated with the same state, uniform, buffers and framebuffer);
+	 * the counters the upload would have zeroed are then cleared by the
+	 * vertex stage. */
+	const uint8_t* hp = reinterpret_cast<const uint8_t*>(R.hParams);
+	const size_t uniformBytes = paramsBytes > kParamsHead ? paramsBytes - kParamsHead : 0;
+	const bool resend = R.sentParams.size() != kTotalsOff + uniformBytes ||
+	                    memcmp(R.sentParams.data(), hp, kTotalsOff) != 0 ||
+	                    (uniformBytes && memcmp(R.sentParams.data() + kTotalsOff, hp + kParamsHead, uniformBytes) != 0);
+	if (resend)
+	{
+		HIP_CHECK(hipMemcpyAsync(R.dParams, &H, paramsBytes, hipMemcpyHostToDevice, R.stream));
+		R.sentParams.assign(hp, hp + kTotalsOff);
+		R.sentParams.insert(R.sentParams.end(), hp + kParamsHead, hp + kParamsHead + uniformBytes);
+	}
 
 	/* Vertex stage (single draw: the multi-draw fields stay zero). */
 	SrpVsLaunch vl;
 	memset(&vl, 0, sizeof(vl));
 	vl.params = R.dParams;
 	vl.count = cacheSize;
+	if (!resend)
+	{
+		vl.zero = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
+		vl.zeroCount = (uint32_t) (kTotalsBytes / sizeof(uint32_t));
+	}
 	vse->launch(&vl, (void*) R.stream);
 
 	/* Assembly COUNT pass + scans. */
 	uint32_t* counts = R.counts.get<uint32_t>((nIn + 1) * 4 * sizeof(uint32_t));
-	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);   /* zeroed by the upload */
+	uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);   /* zeroed by the upload or the vertex stage */
 	Runtime::Pending& D = R.pending;
 	AsmArgs& A = D.A;
 	memset(&A, 0, sizeof(A));
