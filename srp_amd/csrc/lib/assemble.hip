@@ -603,13 +603,38 @@ __device__ __forceinline__ void fanTriangle(const AsmArgs& A, const AsmDraw& W, 
 	}
 }
 
+/* Start of an emitting kernel of a single draw (AsmArgs.fitTotals): false
+ * when it has nothing to do, because the counted totals do not fit the
+ * capacities it was given or nothing was assembled.  Block 0 of the
+ * leading kernel records the outcome and publishes the counters. */
+template <bool EMIT>
+__device__ __forceinline__ bool emitGo(const AsmArgs& A, bool leader)
+{
+	if (!EMIT || !A.fitTotals)
+		return true;
+	uint32_t* t = A.fitTotals;
+	const unsigned long long* t64 = reinterpret_cast<const unsigned long long*>(t + 8);
+	const FitCaps& c = A.fitCaps;
+	const bool fits = t64[0] <= c.records && t64[1] <= c.generated && t64[2] <= c.items &&
+	                  (unsigned long long) t[6] <= c.spans && (unsigned long long) t[4] <= c.clipped;
+	const bool any = t64[0] != 0ull;
+	if (leader && blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		t[7] = !fits ? 1u : (any ? 0u : 2u);
+		t[2] = fits ? (uint32_t) t64[2] : 0u;
+		if (A.fitHost)
+			publish(t, A.fitHost, A.fitSeq);
+	}
+	return fits && any;
+}
+
 /* Triangle draws (all polygon modes): primitive_assembly.c:37-135.  The
  * common case — all three vertices inside the frustum, trivially accepted
  * by clipping.c:74-78 — stays in registers. */
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_tris(AsmArgs A)
 {
-	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+	if (!emitGo<EMIT>(A, true))   /* the counted totals do not fit, or nothing to emit */
 		return;
 	if (!EMIT && A.zero)
 		for (uint32_t z = blockIdx.x * 128u + threadIdx.x; z < A.zeroCount; z += gridDim.x * 128u)
@@ -1173,7 +1198,7 @@ __device__ __forceinline__ void clipGroup(const AsmArgs& A, uint32_t g, bool has
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_clipped(AsmArgs A)
 {
-	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+	if (!emitGo<EMIT>(A, false))   /* the counted totals do not fit, or nothing to emit */
 		return;
 	const uint32_t n = *A.clipCount;
 	constexpr uint32_t perBlock = 128u / CLIP_GROUP;
@@ -1198,7 +1223,7 @@ __device__ __forceinline__ void lineStream(uint32_t prim, uint64_t base, uint64_
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 {
-	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+	if (!emitGo<EMIT>(A, true))   /* the counted totals do not fit, or nothing to emit */
 		return;
 	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
 	const AsmDraw& W = A.draws[d];
@@ -1302,7 +1327,7 @@ __global__ __launch_bounds__(128) void k_assemble_lines(AsmArgs A)
 template <bool EMIT>
 __global__ __launch_bounds__(128) void k_assemble_points(AsmArgs A)
 {
-	if (EMIT && A.skip && *A.skip)   /* the counted totals did not fit (k_fit) */
+	if (!emitGo<EMIT>(A, true))   /* the counted totals do not fit, or nothing to emit */
 		return;
 	const uint32_t d = A.nDraws > 1 ? drawOfBlock(A.draws, A.nDraws, blockIdx.x) : 0u;
 	const AsmDraw& W = A.draws[d];

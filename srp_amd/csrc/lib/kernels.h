@@ -48,6 +48,14 @@ struct SrpSpan
 };
 
 /* Arguments of the assembly kernels (assemble.hip). */
+/* Capacities an emitting pass was sized for before its counts were known,
+ * and the number of triangles its clipping kernels handle (0: they were not
+ * launched, so any clipped triangle makes the draw not fit). */
+struct FitCaps
+{
+	unsigned long long records, generated, items, spans, clipped;
+};
+
 struct AsmArgs
 {
 	const AsmDraw* draws; /* device */
@@ -82,9 +90,20 @@ struct AsmArgs
 	/* capacities of the triangle outputs (records, tile pairs, generated
 	 * vertices); what lies beyond them is dropped and flagged */
 	uint32_t recCap, itemCap, genCap;
-	/* emitting pass: when set and nonzero on the device, the pass does
-	 * nothing (the counted totals did not fit the capacities; see k_fit) */
-	const uint32_t* skip;
+	/* Emitting pass of a single draw: its counter block (u32 words: [0..2]
+	 * the counting pass's totals mod 2^32, [4] clipped triangles, [6] spans,
+	 * [7] the outcome, [8..15] the three exact u64 totals).  Every block
+	 * first checks the totals against fitCaps and does nothing unless they
+	 * fit and there is something to emit; block 0 of the leading kernel
+	 * records the outcome, totals[7] = 0 (fits), 1 (does not fit) or 2
+	 * (nothing to emit), and totals[2] = the item count if it fits, else 0
+	 * (the sort and tile ranges read it on the device), then with fitHost
+	 * publishes the 16 words as launchPublish does.  A draw that does not
+	 * fit thus leaves every buffer unchanged.  nullptr: no check. */
+	uint32_t* fitTotals;
+	FitCaps fitCaps;
+	uint32_t* fitHost;
+	uint32_t fitSeq;
 	/* triangle counting pass: also sets zero[0..zeroCount) to 0 (the tile
 	 * ranges the emitting half fills), saving a launch */
 	uint32_t* zero;
@@ -144,27 +163,18 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
                uint32_t** outKeys, uint32_t** outVals, hipStream_t s, const uint32_t* nDev = nullptr);
 void radixSortReserve(SortScratch& scratch, size_t n);
 
-/* Capacities an emitting pass was sized for before its counts were known,
- * and the number of triangles its clipping kernels handle (0: they were not
- * launched, so any clipped triangle makes the draw not fit). */
-struct FitCaps
-{
-	unsigned long long records, generated, items, spans, clipped;
-};
 /* 16 words of a draw's counter block copied to host-mapped memory
  * (hostCopy: its device address), then hostCopy[16] = seq once they are
  * visible to the host, which waits for that word. */
+__device__ __forceinline__ void publish(const uint32_t* totals, uint32_t* hostCopy, uint32_t seq)
+{
+	for (int k = 0; k < 16; k++)
+		hostCopy[k] = totals[k];
+	__threadfence_system();
+	__atomic_store_n(hostCopy + 16, seq, __ATOMIC_RELEASE);
+}
 void launchPublish(const uint32_t* totals, uint32_t* hostCopy, uint32_t seq, hipStream_t s);
 
-/* The fit check of a single draw's counter block `totals` (u32 words: [0..2]
- * the counting pass's totals mod 2^32, [4] clipped triangles, [6] spans,
- * [7] the skip flag written here; [8..15] the three exact u64 totals): one
- * thread sets totals[7] = 0 if the counts fit `caps`, 1 if they do not, 2
- * if there is nothing to emit, and totals[2] = the item count if they fit,
- * else 0.  The emitting pass, the sort and the tile ranges read these on
- * the device, so a draw that does not fit leaves every buffer unchanged.
- * With hostCopy, the 16 words are then published as by launchPublish. */
-void launchFit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, uint32_t seq, hipStream_t s);
 
 void fillU32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 /* *dst = *src by a one-thread kernel (dst may be host-mapped memory) */

@@ -1296,9 +1296,14 @@ void enqueueEmit(Runtime& R, const FitCaps& want, bool speculative)
 	A.genCap = (uint32_t) std::min<unsigned long long>(caps.generated, 0xFFFFFFFFull);
 	A.spanCap = (uint32_t) caps.spans;
 	A.spanCount = totals + 5;   /* emitting pass: append cursor (zeroed by the upload) */
-	A.skip = totals + 7;
-	uint32_t* items = totals + 2;   /* the item count, or 0 when skipped (k_fit) */
-	launchFit(totals, caps, speculative ? R.dTotals : nullptr, speculative ? ++R.countsSeq : 0u, R.stream);
+	/* the emitting kernels check the totals against these capacities
+	 * themselves; the leading one records the outcome and, for a
+	 * speculative half, publishes the counters to the host */
+	A.fitTotals = totals;
+	A.fitCaps = caps;
+	A.fitHost = speculative ? R.dTotals : nullptr;
+	A.fitSeq = speculative ? ++R.countsSeq : 0u;
+	uint32_t* items = totals + 2;   /* the item count, or 0 when skipped (emitGo) */
 
 	const uint32_t clipped = speculative ? (caps.clipped ? kClipUnknown : 0u) : R.hTotals[4];
 	if (D.kind == 0 && clipped != 0u)
@@ -1443,7 +1448,7 @@ void resolvePending()
 	const uint64_t* h64 = reinterpret_cast<const uint64_t*>(h + 8);
 	if (D.clipSkipped && h[4] != 0u)
 	{
-		/* the emitting half was skipped (k_fit): count again, with the
+		/* the emitting half did nothing (emitGo): count again, with the
 		 * clipping kernel, and wait for those counts */
 		uint32_t* totals = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
 		fillU32(totals + 4, 0u, 3, R.stream);   /* clip list, span cursor, span count */

@@ -600,14 +600,6 @@ void iota(uint32_t* p, size_t n, hipStream_t s)
 	if (n) hipLaunchKernelGGL(k_iota, fillGrid(n), dim3(256), 0, s, p, n);
 }
 
-__device__ __forceinline__ void publish(const uint32_t* totals, uint32_t* hostCopy, uint32_t seq)
-{
-	for (int k = 0; k < 16; k++)
-		hostCopy[k] = totals[k];
-	__threadfence_system();
-	__atomic_store_n(hostCopy + 16, seq, __ATOMIC_RELEASE);
-}
-
 __global__ void k_publish(const uint32_t* totals, uint32_t* hostCopy, uint32_t seq)
 {
 	publish(totals, hostCopy, seq);
@@ -618,22 +610,6 @@ void launchPublish(const uint32_t* totals, uint32_t* hostCopy, uint32_t seq, hip
 	hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, totals, hostCopy, seq);
 }
 
-__global__ void k_fit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, uint32_t seq)
-{
-	const unsigned long long* totals64 = reinterpret_cast<const unsigned long long*>(totals + 8);
-	const bool fits = totals64[0] <= caps.records && totals64[1] <= caps.generated && totals64[2] <= caps.items &&
-	                  (unsigned long long) totals[6] <= caps.spans && (unsigned long long) totals[4] <= caps.clipped;
-	/* 1: does not fit; 2: nothing to emit */
-	totals[7] = !fits ? 1u : (totals64[0] == 0ull ? 2u : 0u);
-	totals[2] = fits ? (uint32_t) totals64[2] : 0u;
-	if (hostCopy)
-		publish(totals, hostCopy, seq);
-}
-
-void launchFit(uint32_t* totals, FitCaps caps, uint32_t* hostCopy, uint32_t seq, hipStream_t s)
-{
-	hipLaunchKernelGGL(k_fit, dim3(1), dim3(1), 0, s, totals, caps, hostCopy, seq);
-}
 
 }  // namespace lib
 }  // namespace srp
