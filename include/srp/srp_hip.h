@@ -103,7 +103,7 @@ int srpHipGetTileDepthCull(void);
  *  same for every setting. */
 void srpHipSetTileOrder(int run);
 int srpHipGetTileOrder(void);
-/** Deferred shading (on by default) of fragment shaders registered with a
+/** Deferred shading (off by default) of fragment shaders registered with a
  *  wave entry (SRP_HIP_FRAGMENT_SHADER_WAVE) that do not write depth, in
  *  single triangle draws with the batched rasteriser: the rasteriser does
  *  every fragment operation but shades nothing, remembering for each pixel
@@ -111,7 +111,9 @@ int srpHipGetTileOrder(void);
  *  then evaluates the shader once per such pixel, 64 pixels per wave, on
  *  the inputs it would have had.  The frame and the fragment counts are
  *  the same either way; off, the shader runs inside the rasteriser for
- *  every fragment that passes the depth test. */
+ *  every fragment that passes the depth test (on the 1080p terrain with
+ *  the learned shader the faster of the two: 0.433 against 0.439 ms per
+ *  frame). */
 void srpHipSetDeferredShading(int enable);
 int srpHipGetDeferredShading(void);
 

@@ -160,7 +160,7 @@ struct Runtime
 	uint32_t bigPixel = 1;  /* larger triangles shaded pixel-parallel in runs */
 	uint32_t depthCull = 1; /* tile-level depth culling in runs */
 	uint32_t tileOrder = 8; /* single draws: runs of this many tiles dealt to the XCDs in turn (0: ranges) */
-	uint32_t deferShading = 1;  /* wave shaders shade each pixel's visible fragment only */
+	uint32_t deferShading = 0;  /* wave shaders shade each pixel's visible fragment only, after the rasteriser */
 	uint32_t lastClipped = 0;   /* triangles the previous draw clipped (grid hint of a new draw) */
 
 	/* A single draw whose emitting half was enqueued before its counts

@@ -280,15 +280,15 @@ def test_compare_matches_numpy(nn, w, h, covered):
 @pytest.mark.parametrize("scale", [1.0, 3.0])
 def test_deferred_shading_equals_shading_in_the_rasteriser(nn, scale):
     """The network as the terrain's fragment shader, evaluated once per
-    visible pixel after the rasteriser (deferred, the default) and for every
-    fragment that passes the depth test inside it: the same colour bits,
+    visible pixel after the rasteriser (deferred) and for every fragment that
+    passes the depth test inside it (the default): the same colour bits,
     depth, stencil and fragment counts, on three poses (each network output
     depends only on its own fragment's inputs, which the resolve rebuilds
     exactly)."""
     W, H = 480, 270
     assert nn.terrain_setup(W, H, 400, 200, 256, 0) == 0
     net = nn.new_net(7)
-    assert nn.get_deferred() == 1
+    assert nn.get_deferred() == 0
     try:
         nn.set_params(net, (no.init_params(7) * scale).astype(np.float32))
         nn.terrain_shader(0, nn.neural_fs, nn.net_device(net))
@@ -308,7 +308,7 @@ def test_deferred_shading_equals_shading_in_the_rasteriser(nn, scale):
             assert sa == sb, (sa, sb)
             assert np.count_nonzero(a[1] < 1) > W * H // 10   # the terrain covers the frame
     finally:
-        nn.set_deferred(1)
+        nn.set_deferred(0)
         nn.free_net(net)
         nn.finish()
 
