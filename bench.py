@@ -50,7 +50,7 @@ def stats(lib, fn):
     return list(a)
 
 
-# The workloads, each a dict of its sizes:
+# The workloads (the benchmark configurations), each a dict of its sizes:
 #   mesh     indexed textured UV sphere (the headline)
 #   tris     random clip-space triangles, 512x512 (the plain-method workload)
 #   mix      clipped triangles + lines + points, 1080p

@@ -14,6 +14,7 @@ import argparse
 import ctypes
 import os
 import sys
+import tempfile
 
 import numpy as np
 
@@ -77,7 +78,7 @@ def check(scene, renderer="gpu", out_dir=None, module=None):
     """Render `scene`, write its frame and compare it with the golden."""
     mod = module or SceneModule(RENDERERS[renderer])
     frame = mod.run(scene)
-    out_dir = out_dir or os.path.join(REPO, "gpurun_out", "images")
+    out_dir = out_dir or os.environ.get("SRP_IMAGE_DIR") or os.path.join(tempfile.gettempdir(), "srp_images")
     out = save_frame(frame.color, os.path.join(out_dir, scene + ".png"))
     return compare(out, os.path.join(GOLDEN_DIR, scene + ".png"), out)
 

@@ -2,7 +2,7 @@
  * library applies to the counting pass's exact 64-bit totals before it
  * sizes buffers or runs the emitting pass: totals at, just past and far
  * beyond 2^31 and 2^32 (synthetic; no draw is run).  Prints one line per
- * case and "ok" when every verdict is the expected one. */
+ * case and "ok" when every answer is the expected one. */
 #include <stdio.h>
 #include <string.h>
 #include "sizing.h"

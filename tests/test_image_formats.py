@@ -11,7 +11,7 @@ here and compared with:
     reference sources are present: every byte must agree;
   - PIL's own decoding (always): every byte must agree, except for the
     16-bit colour formats, where PIL widens 5- and 6-bit channels
-    differently and only stb is the judge.
+    differently and stb's bytes are the ones to match.
 JPEG and GIF are refused with a message (stb would decode them: a
 documented difference)."""
 import ctypes

@@ -46,7 +46,7 @@ def test_oracle_build_equals_compiled_reference(tmp_path):
     if not (cc and os.path.isdir(ref_obj) and os.path.isdir(os.path.join(REF_ROOT, "include"))):
         pytest.skip("compiled reference (oracle/_ref) not built")
     # the reference has no PNG writer of its own in the library (its tests
-    # link stb_image_write); the sample's call is served by this build's
+    # link stb_image_write); the sample's call is served by the library's
     shim = tmp_path / "save.c"
     shim.write_text('#include <srp/srp.h>\n#include "png_encode.h"\n'
                     "int srpFramebufferSaveImage(const SRPFramebuffer* fb, const char* path)\n"
@@ -55,7 +55,7 @@ def test_oracle_build_equals_compiled_reference(tmp_path):
     host = os.path.join(REPO, "srp_amd", "csrc", "host")
     objs = [os.path.join(ref_obj, f) for f in sorted(os.listdir(ref_obj)) if f.endswith(".o")]
     # the reference's own headers first; srp_hip.h (the sample's PNG call)
-    # only exists in this build's include directory
+    # only exists in the library's include directory
     subprocess.run([cc, "-std=c11", "-O2", "-ffp-contract=off", "-I", os.path.join(REF_ROOT, "include"),
                     "-I", os.path.join(REPO, "include"),
                     "-I", os.path.join(BUILD, "src"), "-I", host, os.path.join(BUILD, "src", "main.c"),
