@@ -272,25 +272,37 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_SLICES = 2;     /* slices of RS_THREADS items ranked per round */
 constexpr int RS_CHUNK = 1024;   /* items per block: more blocks than CUs for the 1080p mesh's ~350 k pairs
                                    (measured: 4096 0.234 ms/frame, 2048 0.223, 1024 0.219) */
+/* Items per block of a sort of more than RS_LONG_FROM items: each block
+ * then writes longer runs per digit (RS_LONG_CHUNK / 256 keys on average,
+ * whole cache lines rather than pieces of them), the digit counts to scan
+ * shrink by the same factor, and there are still thousands of blocks. */
+constexpr int RS_LONG_CHUNK = 32768;
+constexpr size_t RS_LONG_FROM = (size_t) 4 << 20;
+
+/* The block size a sort of n items uses (identical for its reservation). */
+inline uint32_t rsChunk(size_t n)
+{
+	return n > RS_LONG_FROM ? (uint32_t) RS_LONG_CHUNK : (uint32_t) RS_CHUNK;
+}
 
 /* Up to this many blocks, each scatter block sums the per-block digit counts
  * itself (reading all of them from L2) instead of a separate scan over them:
  * two launches per pass fewer, at nBlocks KiB of L2 reads per block. */
 constexpr uint32_t RS_FUSED_MAX_BLOCKS = 512;
 
-/* Digit counts of one block of RS_CHUNK keys, written at
+/* Digit counts of one block of `chunk` keys, written at
  * hist[digit * digitStride + block * blockStride]. */
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys, size_t n, const uint32_t* nDev,
                                                            uint32_t shift, uint32_t* hist, uint32_t digitStride,
-                                                           uint32_t blockStride)
+                                                           uint32_t blockStride, uint32_t chunk)
 {
 	if (nDev && *nDev < n) n = *nDev;
 	__shared__ uint32_t h[256];
 	const uint32_t t = threadIdx.x;
 	h[t] = 0;
 	__syncthreads();
-	const size_t begin = (size_t) blockIdx.x * RS_CHUNK;
-	const size_t end = (begin + RS_CHUNK < n) ? begin + RS_CHUNK : n;
+	const size_t begin = (size_t) blockIdx.x * chunk;
+	const size_t end = (begin + chunk < n) ? begin + chunk : n;
 	for (size_t i = begin + t; i < end; i += RS_THREADS)
 		atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
 	__syncthreads();
@@ -306,15 +318,15 @@ template <bool FUSED>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
                                                               uint32_t* keysOut, uint32_t* valsOut, size_t n,
                                                               const uint32_t* nDev, uint32_t shift, const uint32_t* hist,
-                                                              uint32_t nBlocks)
+                                                              uint32_t nBlocks, uint32_t chunk)
 {
 	if (nDev && *nDev < n) n = *nDev;
 	/* blocks past the device count (a capacity-sized grid) have nothing to
 	 * place; whole blocks leave together, before any barrier */
-	if ((size_t) blockIdx.x * RS_CHUNK >= n)
+	if ((size_t) blockIdx.x * chunk >= n)
 		return;
 	/* hist rows of blocks past the count are zero: not read */
-	const uint32_t usedBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
+	const uint32_t usedBlocks = (uint32_t) ((n + chunk - 1) / chunk);
 	__shared__ uint32_t running[256];
 	__shared__ uint32_t sliceCnt[RS_SLICES * 4][256];
 	__shared__ uint32_t tileTotal[256];
@@ -369,8 +381,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 	}
 	else
 		running[t] = hist[(size_t) t * nBlocks + blockIdx.x];
-	const size_t begin = (size_t) blockIdx.x * RS_CHUNK;
-	const size_t end = (begin + RS_CHUNK < n) ? begin + RS_CHUNK : n;
+	const size_t begin = (size_t) blockIdx.x * chunk;
+	const size_t end = (begin + chunk < n) ? begin + chunk : n;
 	const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
 	/* Rounds of RS_SLICES slices of RS_THREADS items; item order is slice,
 	 * then wave, then lane, and so is the order of the per-(slice, wave)
@@ -467,7 +479,8 @@ void radixSortReserve(SortScratch& sc, size_t n)
 {
 	if (n <= 1)
 		return;
-	const uint32_t nBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
+	const uint32_t chunk = rsChunk(n);
+	const uint32_t nBlocks = (uint32_t) ((n + chunk - 1) / chunk);
 	if (n > sc.cap)
 	{
 		if (sc.keys) hipFree(sc.keys);
@@ -494,7 +507,8 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 	*outVals = vals;
 	if (n <= 1 || bits <= 0)
 		return;
-	const uint32_t nBlocks = (uint32_t) ((n + RS_CHUNK - 1) / RS_CHUNK);
+	const uint32_t chunk = rsChunk(n);
+	const uint32_t nBlocks = (uint32_t) ((n + chunk - 1) / chunk);
 	radixSortReserve(sc, n);
 	const size_t histN = (size_t) 256 * nBlocks;
 	uint32_t* kIn = keys;
@@ -507,17 +521,17 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 		if (fused)
 		{
 			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
-			                   sc.hist, 1u, 256u);
+			                   sc.hist, 1u, 256u, chunk);
 			hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
-			                   nDev, (uint32_t) shift, sc.hist, nBlocks);
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks, chunk);
 		}
 		else
 		{
 			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
-			                   sc.hist, nBlocks, 1u);
+			                   sc.hist, nBlocks, 1u, chunk);
 			scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
 			hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
-			                   nDev, (uint32_t) shift, sc.hist, nBlocks);
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks, chunk);
 		}
 		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
 		uint32_t* tv = vIn; vIn = vOut; vOut = tv;
