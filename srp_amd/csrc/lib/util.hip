@@ -106,49 +106,86 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(uint32_t* base, si
                                                               unsigned long long* sums64, uint32_t nb)
 {
 	uint32_t* data = base + (size_t) blockIdx.y * stride;
-	__shared__ uint32_t sh[SCAN_THREADS];
+	__shared__ uint32_t waveTot[SCAN_THREADS / 64];
 	__shared__ unsigned long long waveSum[SCAN_THREADS / 64];
-	const uint32_t t = threadIdx.x;
+	const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
 	const size_t b0 = (size_t) blockIdx.x * SCAN_CHUNK + (size_t) t * SCAN_ITEMS;
+	/* this thread's SCAN_ITEMS consecutive values: two 16-byte loads when
+	 * they are all there and aligned (the arrays of one launch start at
+	 * arbitrary word offsets) */
+	const bool vec = b0 + SCAN_ITEMS <= n && (reinterpret_cast<uintptr_t>(data + b0) & 15u) == 0;
 	uint32_t v[SCAN_ITEMS];
+	if (vec)
+	{
+		const uint4 a = *reinterpret_cast<const uint4*>(data + b0);
+		const uint4 b = *reinterpret_cast<const uint4*>(data + b0 + 4);
+		v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+		v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+	}
+	else
+	{
+#pragma unroll
+		for (int j = 0; j < SCAN_ITEMS; j++)
+			v[j] = (b0 + j < n) ? data[b0 + j] : 0u;
+	}
 	uint32_t sum = 0;
 	unsigned long long wide = 0;
 #pragma unroll
 	for (int j = 0; j < SCAN_ITEMS; j++)
 	{
-		const size_t i = b0 + j;
-		v[j] = (i < n) ? data[i] : 0u;
 		sum += v[j];
 		wide += v[j];
 	}
-	/* The block's sum without wrap-around: the offsets below are exact only
-	 * while the array's total fits 32 bits, which the caller checks. */
+	/* inclusive scan of the threads' sums within the wave, then across the
+	 * block's waves.  (The block's sum may wrap: the offsets below are
+	 * exact only while the array's total fits 32 bits, which the caller
+	 * checks; the 64-bit sums are exact.) */
+	uint32_t incl = sum;
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1)
+	{
+		const uint32_t x = __shfl_up(incl, off);
+		if ((int) lane >= off) incl += x;
+	}
 	for (int off = 32; off > 0; off >>= 1)
 		wide += __shfl_xor(wide, off);
-	if ((t & 63u) == 0)
-		waveSum[t >> 6] = wide;
-	sh[t] = sum;
-	__syncthreads();
-	for (uint32_t off = 1; off < SCAN_THREADS; off <<= 1)
+	if (lane == 63u)
 	{
-		const uint32_t x = (t >= off) ? sh[t - off] : 0u;
-		__syncthreads();
-		sh[t] += x;
-		__syncthreads();
+		waveTot[wave] = incl;
+		waveSum[wave] = wide;
 	}
-	uint32_t run = sh[t] - sum;
+	__syncthreads();
+	uint32_t run = incl - sum;
+	for (uint32_t w = 0; w < wave; w++)
+		run += waveTot[w];
 #pragma unroll
 	for (int j = 0; j < SCAN_ITEMS; j++)
 	{
-		const size_t i = b0 + j;
-		if (i < n) data[i] = run;
-		run += v[j];
+		const uint32_t x = v[j];
+		v[j] = run;
+		run += x;
+	}
+	if (vec)
+	{
+		*reinterpret_cast<uint4*>(data + b0) = make_uint4(v[0], v[1], v[2], v[3]);
+		*reinterpret_cast<uint4*>(data + b0 + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+	}
+	else
+	{
+#pragma unroll
+		for (int j = 0; j < SCAN_ITEMS; j++)
+			if (b0 + j < n) data[b0 + j] = v[j];
 	}
 	if (t == SCAN_THREADS - 1)
 	{
-		sums[(size_t) blockIdx.y * nb + blockIdx.x] = sh[t];
+		uint32_t bs = 0;
 		unsigned long long w = 0;
-		for (int k = 0; k < SCAN_THREADS / 64; k++) w += waveSum[k];
+		for (int k = 0; k < SCAN_THREADS / 64; k++)
+		{
+			bs += waveTot[k];
+			w += waveSum[k];
+		}
+		sums[(size_t) blockIdx.y * nb + blockIdx.x] = bs;
 		sums64[(size_t) blockIdx.y * nb + blockIdx.x] = w;
 	}
 }
@@ -160,18 +197,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_finish(uint32_t* base, si
 {
 	uint32_t* data = base + (size_t) blockIdx.y * stride;
 	const uint32_t* s = sums + (size_t) blockIdx.y * nb;
-	__shared__ uint32_t red[SCAN_THREADS];
-	const uint32_t t = threadIdx.x;
+	__shared__ uint32_t red[SCAN_THREADS / 64];
+	const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
 	uint32_t acc = 0;
 	for (uint32_t k = t; k < blockIdx.x; k += SCAN_THREADS) acc += s[k];
-	red[t] = acc;
+	for (int off = 32; off > 0; off >>= 1)
+		acc += __shfl_xor(acc, off);
+	if (lane == 0)
+		red[wave] = acc;
 	__syncthreads();
-	for (uint32_t off = SCAN_THREADS / 2; off > 0; off >>= 1)
-	{
-		if (t < off) red[t] += red[t + off];
-		__syncthreads();
-	}
-	const uint32_t add = red[0];
+	uint32_t add = 0;
+	for (int w = 0; w < SCAN_THREADS / 64; w++)
+		add += red[w];
 	const size_t b0 = (size_t) blockIdx.x * SCAN_CHUNK;
 	if (add)
 		for (int j = 0; j < SCAN_ITEMS; j++)
@@ -186,19 +223,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_finish(uint32_t* base, si
 		if (totals64)
 		{
 			/* the array's exact total (the last block reduces every block's) */
-			__shared__ unsigned long long red64[SCAN_THREADS];
+			__shared__ unsigned long long red64[SCAN_THREADS / 64];
 			const unsigned long long* s64 = sums64 + (size_t) blockIdx.y * nb;
 			unsigned long long acc64 = 0;
 			for (uint32_t k = t; k < nb; k += SCAN_THREADS) acc64 += s64[k];
-			red64[t] = acc64;
+			for (int off = 32; off > 0; off >>= 1)
+				acc64 += __shfl_xor(acc64, off);
+			if (lane == 0)
+				red64[wave] = acc64;
 			__syncthreads();
-			for (uint32_t off = SCAN_THREADS / 2; off > 0; off >>= 1)
-			{
-				if (t < off) red64[t] += red64[t + off];
-				__syncthreads();
-			}
 			if (t == 0)
-				totals64[blockIdx.y] = red64[0];
+			{
+				unsigned long long tot = 0;
+				for (int w = 0; w < SCAN_THREADS / 64; w++)
+					tot += red64[w];
+				totals64[blockIdx.y] = tot;
+			}
 		}
 	}
 }
