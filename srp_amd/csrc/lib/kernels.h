@@ -152,6 +152,7 @@ struct SortScratch
 	uint32_t* keys = nullptr;
 	uint32_t* vals = nullptr;
 	uint32_t* hist = nullptr;
+	uint32_t* grp = nullptr;   /* per-group digit sums and a counter (fused passes) */
 	size_t cap = 0, histCap = 0;
 	ScanScratch scan;
 };

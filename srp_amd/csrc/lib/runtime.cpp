@@ -1107,6 +1107,37 @@ void srp_lib_scan_exact(const uint32_t* counts, size_t n, int nArrays, uint32_t*
 	if (sc.buf64) HIP_CHECK(hipFree(sc.buf64));
 }
 
+/* Test hook (not part of the public headers): the stable radix sort the
+ * draws use, on host data, with the runtime's own sort scratch.  keys and
+ * vals hold cap items; the first count of them (count <= cap, read on the
+ * device as the draws pass it) are sorted on the low `bits` bits of the
+ * keys, in place. */
+void srp_lib_radix_sort(uint32_t* keys, uint32_t* vals, size_t cap, size_t count, int bits)
+{
+	resolvePending();
+	Runtime& R = rt();
+	uint32_t* dk = nullptr;
+	uint32_t* dv = nullptr;
+	uint32_t* dn = nullptr;
+	const size_t bytes = cap * sizeof(uint32_t);
+	HIP_CHECK(hipMalloc((void**) &dk, bytes + 16));
+	HIP_CHECK(hipMalloc((void**) &dv, bytes + 16));
+	HIP_CHECK(hipMalloc((void**) &dn, sizeof(uint32_t)));
+	const uint32_t count32 = (uint32_t) count;
+	HIP_CHECK(hipMemcpyAsync(dk, keys, bytes, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(dv, vals, bytes, hipMemcpyHostToDevice, R.stream));
+	HIP_CHECK(hipMemcpyAsync(dn, &count32, sizeof(uint32_t), hipMemcpyHostToDevice, R.stream));
+	uint32_t* ok = nullptr;
+	uint32_t* ov = nullptr;
+	radixSort(dk, dv, cap, bits, R.sort, &ok, &ov, R.stream, dn);
+	HIP_CHECK(hipMemcpyAsync(keys, ok, count * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipMemcpyAsync(vals, ov, count * sizeof(uint32_t), hipMemcpyDeviceToHost, R.stream));
+	HIP_CHECK(hipStreamSynchronize(R.stream));
+	HIP_CHECK(hipFree(dk));
+	HIP_CHECK(hipFree(dv));
+	HIP_CHECK(hipFree(dn));
+}
+
 void srpHipSetTileDepthCull(int enable)
 {
 	resolvePending();
@@ -2383,7 +2414,7 @@ void srpFreeBatch(SRPBatch* B)
 	batchDropGraph(B);
 	void* bufs[] = {B->dBlocks, B->dDraws, B->dVsSeg, B->dRasterSeg, B->clip, B->varyings, B->counts, B->totals,
 	                B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings, B->tileRange,
-	                B->queues, B->scan.buf, B->scan.buf64, B->sort.keys, B->sort.vals, B->sort.hist,
+	                B->queues, B->scan.buf, B->scan.buf64, B->sort.keys, B->sort.vals, B->sort.hist, B->sort.grp,
 	                B->sort.scan.buf, B->sort.scan.buf64};
 	for (void* p : bufs)
 		batchFreeDev(p);

@@ -330,11 +330,21 @@ inline uint32_t rsChunk(size_t n)
  * two launches per pass fewer, at nBlocks KiB of L2 reads per block. */
 constexpr uint32_t RS_FUSED_MAX_BLOCKS = 512;
 
+/* Up to RS_FUSED_MAX_BLOCKS blocks, the digit counts are also summed per
+ * group of RS_GROUP consecutive blocks (grp[group * 256 + digit], atomics),
+ * so that a scatter block finds its offsets from the group rows and at most
+ * RS_GROUP - 1 block rows.  The groups' sums are zero when a pass starts:
+ * the last scatter block of each pass to read them clears them (and the
+ * pass's counter of such blocks, grp[RS_GROUP_WORDS]). */
+constexpr uint32_t RS_GROUP = 16;
+constexpr uint32_t RS_GROUP_WORDS = RS_FUSED_MAX_BLOCKS / RS_GROUP * 256u;
+
 /* Digit counts of one block of `chunk` keys, written at
- * hist[digit * digitStride + block * blockStride]. */
+ * hist[digit * digitStride + block * blockStride]; with grp, also added to
+ * the block's group row. */
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys, size_t n, const uint32_t* nDev,
                                                            uint32_t shift, uint32_t* hist, uint32_t digitStride,
-                                                           uint32_t blockStride, uint32_t chunk)
+                                                           uint32_t blockStride, uint32_t chunk, uint32_t* grp)
 {
 	if (nDev && *nDev < n) n = *nDev;
 	__shared__ uint32_t h[256];
@@ -347,18 +357,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* keys,
 		atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
 	__syncthreads();
 	hist[(size_t) t * digitStride + (size_t) blockIdx.x * blockStride] = h[t];
+	if (grp && h[t])
+		atomicAdd(grp + (blockIdx.x / RS_GROUP) * 256u + t, h[t]);
 }
 
 /* Stable scatter: items keep their input order within a digit.  Each tile
  * of 256 items ranks its items per digit with wave ballots, then by wave.
- * FUSED: hist holds raw block-major counts (hist[block * 256 + digit]) and
- * the block finds its digits' offsets itself; otherwise hist holds the
- * exclusive scan of the digit-major counts. */
+ * FUSED: hist holds raw block-major counts (hist[block * 256 + digit]), grp
+ * their sums per group of blocks, and the block finds its digits' offsets
+ * itself; otherwise hist holds the exclusive scan of the digit-major
+ * counts. */
 template <bool FUSED>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
                                                               uint32_t* keysOut, uint32_t* valsOut, size_t n,
                                                               const uint32_t* nDev, uint32_t shift, const uint32_t* hist,
-                                                              uint32_t nBlocks, uint32_t chunk)
+                                                              uint32_t nBlocks, uint32_t chunk, uint32_t* grp)
 {
 	if (nDev && *nDev < n) n = *nDev;
 	/* blocks past the device count (a capacity-sized grid) have nothing to
@@ -370,29 +383,40 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 	__shared__ uint32_t running[256];
 	__shared__ uint32_t sliceCnt[RS_SLICES * 4][256];
 	__shared__ uint32_t tileTotal[256];
+	__shared__ bool lastReader;
 	uint32_t (*waveCnt)[256] = sliceCnt;   /* the offsets prologue's per-wave sums */
 	const uint32_t t = threadIdx.x;
 	const uint32_t lane = t & 63u, wave = t >> 6;
 	if (FUSED)
 	{
 		/* Digit t: items of all blocks (total) and of the blocks before
-		 * this one (before).  Wave w reads every fourth block row, a lane
-		 * four digits at once; the four waves' sums meet in LDS. */
+		 * this one (before), from the group rows and then the block rows
+		 * of this block's group before it.  Wave w reads every fourth row,
+		 * a lane four digits at once; the four waves' sums meet in LDS. */
 		__shared__ uint32_t befCnt[4][256];
 		const uint32_t d4 = 4u * lane;
+		const uint32_t myGroup = blockIdx.x / RS_GROUP;
+		const uint32_t nGroups = (usedBlocks + RS_GROUP - 1) / RS_GROUP;
 		uint32_t tot[4] = {0u, 0u, 0u, 0u}, bef[4] = {0u, 0u, 0u, 0u};
-#pragma unroll 8
-		for (uint32_t b = wave; b < usedBlocks; b += 4u)
+		for (uint32_t g = wave; g < nGroups; g += 4u)
 		{
-			const uint4 v = *reinterpret_cast<const uint4*>(hist + (size_t) b * 256u + d4);
+			const uint4 v = *reinterpret_cast<const uint4*>(grp + (size_t) g * 256u + d4);
 			const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
-			const uint32_t take = (b < blockIdx.x) ? ~0u : 0u;
+			const uint32_t take = (g < myGroup) ? ~0u : 0u;
 #pragma unroll
 			for (int k = 0; k < 4; k++)
 			{
 				tot[k] += vv[k];
 				bef[k] += vv[k] & take;
 			}
+		}
+		for (uint32_t b = myGroup * RS_GROUP + wave; b < blockIdx.x; b += 4u)
+		{
+			const uint4 v = *reinterpret_cast<const uint4*>(hist + (size_t) b * 256u + d4);
+			bef[0] += v.x;
+			bef[1] += v.y;
+			bef[2] += v.z;
+			bef[3] += v.w;
 		}
 #pragma unroll
 		for (int k = 0; k < 4; k++)
@@ -417,6 +441,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 		for (uint32_t w = 0; w < wave; w++)
 			waveBase += tileTotal[w];
 		running[t] = waveBase + incl - total + before;
+		/* Every group row this block reads has been read (the sums above
+		 * are done); the block that finishes reading last clears the rows
+		 * and the counter for the next pass. */
+		if (t == 0)
+			lastReader = atomicAdd(grp + RS_GROUP_WORDS, 1u) == usedBlocks - 1u;
 		__syncthreads();
 	}
 	else
@@ -484,6 +513,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* ke
 		__syncthreads();
 		running[t] += tileTotal[t];
 	}
+	if (FUSED && lastReader)
+	{
+		const uint32_t nGroups = (usedBlocks + RS_GROUP - 1) / RS_GROUP;
+		for (uint32_t i = t; i < nGroups * 256u; i += RS_THREADS)
+			grp[i] = 0u;
+		if (t == 0)
+			grp[RS_GROUP_WORDS] = 0u;
+	}
 }
 
 /* Scratch of a multi-array scan of nArrays x n values, allocated now. */
@@ -538,6 +575,14 @@ void radixSortReserve(SortScratch& sc, size_t n)
 	}
 	if (nBlocks > RS_FUSED_MAX_BLOCKS)
 		scanReserve(sc.scan, histN, 1);
+	else if (!sc.grp)
+	{
+		/* zeroed once (a copy, complete on return); every fused pass leaves
+		 * it zero again */
+		static const uint32_t zeros[RS_GROUP_WORDS + 1] = {};
+		hipMalloc(&sc.grp, sizeof(zeros));
+		hipMemcpy(sc.grp, zeros, sizeof(zeros), hipMemcpyHostToDevice);
+	}
 }
 
 void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& sc,
@@ -561,17 +606,17 @@ void radixSort(uint32_t* keys, uint32_t* vals, size_t n, int bits, SortScratch& 
 		if (fused)
 		{
 			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
-			                   sc.hist, 1u, 256u, chunk);
+			                   sc.hist, 1u, 256u, chunk, sc.grp);
 			hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
-			                   nDev, (uint32_t) shift, sc.hist, nBlocks, chunk);
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks, chunk, sc.grp);
 		}
 		else
 		{
 			hipLaunchKernelGGL(k_radix_hist, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, n, nDev, (uint32_t) shift,
-			                   sc.hist, nBlocks, 1u, chunk);
+			                   sc.hist, nBlocks, 1u, chunk, (uint32_t*) nullptr);
 			scanExclusiveMulti(sc.hist, histN, 1, histN, sc.hist + histN, sc.scan, s);
 			hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nBlocks), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n,
-			                   nDev, (uint32_t) shift, sc.hist, nBlocks, chunk);
+			                   nDev, (uint32_t) shift, sc.hist, nBlocks, chunk, (uint32_t*) nullptr);
 		}
 		uint32_t* tk = kIn; kIn = kOut; kOut = tk;
 		uint32_t* tv = vIn; vIn = vOut; vOut = tv;
