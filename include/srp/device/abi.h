@@ -11,7 +11,7 @@
 #include <stdint.h>
 #include "srp/context.h"
 
-#define SRP_ABI_VERSION 15u
+#define SRP_ABI_VERSION 16u
 #define SRP_MAX_VARYINGS 16
 #define SRP_MAX_WORDS 32          /* fast interpolation path: 4-byte words */
 #define SRP_MAX_VARYING_BYTES 256 /* general path: bytes per fragment */
@@ -172,6 +172,15 @@ typedef struct SrpVsLaunch
 	 * the draw's counters, when its parameters were not uploaded again */
 	uint32_t* zero;
 	uint32_t zeroCount;
+	/* single draws: the framebuffer's pending fill rides along in fillBlocks
+	 * blocks after the vertex blocks, setting fillCount pixels of each
+	 * non-null array (depth as its bits, stencil bytes) to the value given */
+	uint32_t* fillColor;
+	uint32_t* fillDepth;
+	uint8_t* fillStencil;
+	uint32_t fillColorValue, fillDepthValue, fillStencilValue;
+	uint32_t fillBlocks;
+	uint64_t fillCount;
 } SrpVsLaunch;
 
 typedef struct SrpRasterLaunch

@@ -327,13 +327,48 @@ __device__ __forceinline__ void flushStats(const SrpDrawParams* P, uint32_t cove
 
 /* ------------------------------------------------------------- kernels */
 
+/* Block b of a vertex launch's framebuffer fill (SrpVsLaunch.fill*). */
+__device__ __forceinline__ void fillBlock(const SrpVsLaunch& L, uint32_t b)
+{
+	const size_t n = L.fillCount;
+	const size_t i0 = (size_t) b * 256u + threadIdx.x;
+	const size_t stride = (size_t) L.fillBlocks * 256u;
+	for (size_t i = i0; i < n; i += stride)
+	{
+		if (L.fillColor) L.fillColor[i] = L.fillColorValue;
+		if (L.fillDepth) L.fillDepth[i] = L.fillDepthValue;
+	}
+	if (L.fillStencil)
+	{
+		/* four bytes at a time when aligned, then the rest */
+		const uint32_t v = (L.fillStencilValue & 255u) * 0x01010101u;
+		size_t done = 0;
+		if (((uintptr_t) L.fillStencil & 3u) == 0)
+		{
+			const size_t n4 = n / 4;
+			for (size_t i = i0; i < n4; i += stride)
+				reinterpret_cast<uint32_t*>(L.fillStencil)[i] = v;
+			done = n4 * 4;
+		}
+		for (size_t i = done + i0; i < n; i += stride)
+			L.fillStencil[i] = (uint8_t) v;
+	}
+}
+
 /* Vertex stage: one thread per post-VS cache slot (vertex index
- * cacheBase + i).  reference pipeline/vertex_processing.c:55-74. */
+ * cacheBase + i).  reference pipeline/vertex_processing.c:55-74.  Blocks
+ * past the vertex blocks fill the framebuffer (fillBlock). */
 template <class VS>
 __global__ __launch_bounds__(256) void srp_vs_kernel(SrpVsLaunch L)
 {
 	if (L.zero && blockIdx.x == 0 && threadIdx.x < L.zeroCount)
 		L.zero[threadIdx.x] = 0u;
+	const uint32_t vsBlocks = L.nSeg ? L.nBlocks : (L.count + 255u) / 256u;
+	if (blockIdx.x >= vsBlocks)
+	{
+		fillBlock(L, blockIdx.x - vsBlocks);
+		return;
+	}
 	const SrpDrawParams* P = L.params;
 	uint32_t i = blockIdx.x * 256u + threadIdx.x;
 	uint32_t count = L.count;
@@ -1415,7 +1450,7 @@ __global__ __launch_bounds__(256) void srp_frag_kernel(SrpFragLaunch L)
 template <class VS>
 void launchVS(const SrpVsLaunch* L, void* stream)
 {
-	const uint32_t blocks = L->nSeg ? L->nBlocks : (L->count + 255u) / 256u;
+	const uint32_t blocks = (L->nSeg ? L->nBlocks : (L->count + 255u) / 256u) + L->fillBlocks;
 	if (blocks)
 		hipLaunchKernelGGL(srp_vs_kernel<VS>, dim3(blocks), dim3(256), 0, (hipStream_t) stream, *L);
 }

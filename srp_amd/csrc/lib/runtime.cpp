@@ -1693,10 +1693,10 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	H.tilesY = (H.height + R.tile - 1) / R.tile;
 	H.stats = R.dStats;
 
+	/* a deferred framebuffer's pending fill rides along with the vertex
+	 * stage below (nothing before the rasteriser touches the framebuffer) */
 	if (!F->deferred)
 		fbUpload(F);
-	else
-		fbFlushFills(F);
 	hipEvent_t evDraw = (R.prof(1) || R.prof(3)) ? R.mark() : nullptr;
 	{
 		AsmDraw& AD = *reinterpret_cast<AsmDraw*>((uint8_t*) R.hParams + kAsmDrawOff);
@@ -1736,6 +1736,18 @@ void drawBuffer(const SRPIndexBuffer* ib, const SRPVertexBuffer* vb, const SRPFr
 	{
 		vl.zero = reinterpret_cast<uint32_t*>((uint8_t*) R.dParams + kTotalsOff);
 		vl.zeroCount = (uint32_t) (kTotalsBytes / sizeof(uint32_t));
+	}
+	if (F->deferred && (F->pendColor || F->pendDepth || F->pendStencil))
+	{
+		vl.fillColor = F->pendColor ? F->dColor : nullptr;
+		vl.fillDepth = F->pendDepth ? (uint32_t*) F->dDepth : nullptr;
+		vl.fillStencil = F->pendStencil ? F->dStencil : nullptr;
+		vl.fillColorValue = F->fillColor;
+		vl.fillDepthValue = F->fillDepthBits;
+		vl.fillStencilValue = F->fillStencil;
+		vl.fillCount = F->pub.size;
+		vl.fillBlocks = (uint32_t) std::min<size_t>((F->pub.size + 1023) / 1024, 2048);
+		F->pendColor = F->pendDepth = F->pendStencil = false;
 	}
 	vse->launch(&vl, (void*) R.stream);
 
