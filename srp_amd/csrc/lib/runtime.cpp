@@ -1115,6 +1115,8 @@ void srp_lib_scan_exact(const uint32_t* counts, size_t n, int nArrays, uint32_t*
 void srp_lib_radix_sort(uint32_t* keys, uint32_t* vals, size_t cap, size_t count, int bits)
 {
 	resolvePending();
+	if (count > cap)
+		count = cap;
 	Runtime& R = rt();
 	uint32_t* dk = nullptr;
 	uint32_t* dv = nullptr;

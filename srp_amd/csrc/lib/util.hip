@@ -9,6 +9,7 @@ namespace lib {
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+static_assert(SCAN_ITEMS == 8, "k_scan_blocks loads a thread's values as two 16-byte vectors");
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_local(uint32_t* data, size_t n, uint32_t* sums)
 {
