@@ -184,6 +184,22 @@ void fillU8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void iota(uint32_t* p, size_t n, hipStream_t s);
 void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv, uint8_t* stencil, uint8_t sv, size_t n,
                      hipStream_t s);
+/* One framebuffer of a multi-framebuffer fill: its arrays (null: not
+ * filled), values, pixels and first block in the launch. */
+struct FillDesc
+{
+	uint32_t* color;
+	uint32_t* depth;
+	uint8_t* stencil;
+	uint32_t colorValue, depthValue, stencilValue;
+	uint32_t firstBlock;
+	uint64_t n;
+};
+/* Blocks a framebuffer of n pixels gets in fillFramebuffers. */
+uint32_t fillBlocksFor(size_t n);
+/* Every framebuffer of desc (device, nDesc entries, blocks in all) in one
+ * launch. */
+void fillFramebuffers(const FillDesc* desc, uint32_t nDesc, uint32_t blocks, hipStream_t s);
 
 /* runtime.cpp: the library stream, and the device arrays of a framebuffer
  * (pending deferred fills applied first; a synchronous framebuffer's host

@@ -1911,6 +1911,8 @@ struct SRPBatch
 	uint32_t nClipped = 0;                   /* triangles clipped at sizing (grid hint) */
 	uint32_t* tileRange = nullptr;           /* start[totalTiles], end[totalTiles] */
 	uint32_t* queues = nullptr;
+	FillDesc* dFills = nullptr;              /* the clears as one fill launch */
+	uint32_t fillBlocks = 0;
 	ScanScratch scan;
 	SortScratch sort;
 	SRPBatchTuning tuning = {8, SRP_SMALL_BBOX, 1, 1, 1, 8};
@@ -1953,6 +1955,29 @@ void batchUpload(SRPBatch* B)
 	HIP_CHECK(hipMemcpyAsync(B->dDraws, B->hDraws.data(), nd * sizeof(AsmDraw), hipMemcpyHostToDevice, R.stream));
 	HIP_CHECK(hipMemcpyAsync(B->dVsSeg, B->hVsSeg.data(), nd * sizeof(SrpSeg), hipMemcpyHostToDevice, R.stream));
 	HIP_CHECK(hipMemcpyAsync(B->dRasterSeg, B->hRasterSeg.data(), nd * sizeof(SrpSeg), hipMemcpyHostToDevice, R.stream));
+	/* the clears, as one fill launch over every cleared framebuffer */
+	std::vector<FillDesc> fills;
+	uint32_t blocks = 0;
+	for (const SRPBatch::Clear& c : B->clears)
+	{
+		FillDesc f;
+		f.color = c.F->dColor;
+		f.depth = (uint32_t*) c.F->dDepth;
+		f.stencil = c.F->dStencil;
+		f.colorValue = 0u;
+		f.depthValue = c.depthBits;
+		f.stencilValue = c.stencil;
+		f.firstBlock = blocks;
+		f.n = c.F->pub.size;
+		blocks += fillBlocksFor(f.n);
+		fills.push_back(f);
+	}
+	if (!B->dFills && !fills.empty())
+		HIP_CHECK(hipMalloc((void**) &B->dFills, B->clears.size() * sizeof(FillDesc)));
+	if (!fills.empty())
+		HIP_CHECK(hipMemcpyAsync(B->dFills, fills.data(), fills.size() * sizeof(FillDesc), hipMemcpyHostToDevice,
+		                         R.stream));
+	B->fillBlocks = blocks;
 	HIP_CHECK(hipStreamSynchronize(R.stream));
 }
 
@@ -2088,8 +2113,7 @@ bool batchSize(SRPBatch* B, const char* caller)
  * synchronisation, fills by kernels). */
 void batchLaunch(SRPBatch* B, hipStream_t s)
 {
-	for (const SRPBatch::Clear& c : B->clears)
-		fillFramebuffer(c.F->dColor, 0u, (uint32_t*) c.F->dDepth, c.depthBits, c.F->dStencil, c.stencil, c.F->pub.size, s);
+	fillFramebuffers(B->dFills, (uint32_t) B->clears.size(), B->fillBlocks, s);
 	batchFront(B, s);
 	AsmArgs A = batchAsmArgs(B, true);
 	launchAssemble(A, 0, true, s, kClipUnknown, B->nClipped);
@@ -2428,7 +2452,7 @@ void srpFreeBatch(SRPBatch* B)
 	batchDropGraph(B);
 	void* bufs[] = {B->dBlocks, B->dDraws, B->dVsSeg, B->dRasterSeg, B->clip, B->varyings, B->counts, B->totals,
 	                B->tris, B->spans, B->pairKeys, B->pairVals, B->pairRec, B->pairMask, B->genVaryings, B->tileRange,
-	                B->queues, B->scan.buf, B->scan.buf64, B->sort.keys, B->sort.vals, B->sort.hist, B->sort.grp,
+	                B->queues, B->scan.buf, B->scan.buf64, B->sort.keys, B->sort.vals, B->sort.hist, B->sort.grp, B->dFills,
 	                B->sort.scan.buf, B->sort.scan.buf64};
 	for (void* p : bufs)
 		batchFreeDev(p);

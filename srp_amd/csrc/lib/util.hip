@@ -695,6 +695,41 @@ void fillFramebuffer(uint32_t* color, uint32_t cv, uint32_t* depth, uint32_t dv,
 	if (n) hipLaunchKernelGGL(k_fill_fb, fillGrid(n), dim3(256), 0, s, color, cv, depth, dv, stencil, sv, n);
 }
 
+/* Block b fills framebuffer d, the last descriptor with firstBlock <= b,
+ * with the blocks from its firstBlock to the next one's. */
+__global__ void k_fill_many(const FillDesc* desc, uint32_t nDesc, uint32_t blocks)
+{
+	uint32_t a = 0, z = nDesc;
+	while (z - a > 1)
+	{
+		const uint32_t m = (a + z) >> 1;
+		if (desc[m].firstBlock <= blockIdx.x) a = m;
+		else z = m;
+	}
+	const FillDesc D = desc[a];
+	const uint32_t nb = ((a + 1 < nDesc) ? desc[a + 1].firstBlock : blocks) - D.firstBlock;
+	const size_t i0 = (size_t) (blockIdx.x - D.firstBlock) * blockDim.x + threadIdx.x;
+	const size_t stride = (size_t) nb * blockDim.x;
+	for (size_t i = i0; i < D.n; i += stride)
+	{
+		if (D.color) D.color[i] = D.colorValue;
+		if (D.depth) D.depth[i] = D.depthValue;
+		if (D.stencil) D.stencil[i] = (uint8_t) D.stencilValue;
+	}
+}
+
+uint32_t fillBlocksFor(size_t n)
+{
+	const size_t b = (n + 1023) / 1024;
+	return (uint32_t) (b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+void fillFramebuffers(const FillDesc* desc, uint32_t nDesc, uint32_t blocks, hipStream_t s)
+{
+	if (nDesc && blocks)
+		hipLaunchKernelGGL(k_fill_many, dim3(blocks), dim3(256), 0, s, desc, nDesc, blocks);
+}
+
 void iota(uint32_t* p, size_t n, hipStream_t s)
 {
 	if (n) hipLaunchKernelGGL(k_iota, fillGrid(n), dim3(256), 0, s, p, n);
